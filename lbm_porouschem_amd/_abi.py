@@ -1,0 +1,252 @@
+"""ctypes binding of the C ABI declared in include/lbm_porouschem.h.
+
+`load_library()` opens the product library liblbm_mi355x.so (HIP kernels for
+gfx950) that sits next to this file.  There is no fallback: if the library is
+missing the import of the solver fails with an error naming the build step.
+`CLib` can also wrap another library exporting the same ABI (the tests use this
+to drive the CPU checker under oracle/ with identical calls).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ABI_VERSION = 1
+MAX_SPECIES = 16
+MAX_REACTIONS = 16
+MAX_FACES = 6
+
+# status codes
+OK = 0
+ERR_INVALID = -1
+ERR_UNSUPPORTED = -2
+ERR_OOM = -3
+ERR_DEVICE = -4
+ERR_DIVERGED = -5
+ERR_STATE = -6
+
+PREC_FP64, PREC_FP32, PREC_FP16 = 0, 1, 2
+COLL_BGK, COLL_MRT = 0, 1
+
+FIELD_RHO = 0
+FIELD_VEL = 1
+FIELD_SOLID = 2
+FIELD_RHOS = 3
+FIELD_COEF_DARCY = 4
+FIELD_COEF_FORCH = 5
+FIELD_TF = 6
+FIELD_TS = 7
+FIELD_TF_DS = 8
+FIELD_TS_DS = 9
+FIELD_EXCH_SURF = 10
+FIELD_EXCH_COEF = 11
+FIELD_RAD_SURF = 12
+FIELD_EMISSIVITY = 13
+FIELD_SPECIES = 14
+FIELD_SPECIES_DS = 15
+FIELD_V_BC_PROFILE = 16
+FIELD_RHO_BC_PROFILE = 17
+
+DTYPE_F64, DTYPE_F32 = 0, 1
+
+REDUCE_MAX_V = 0
+REDUCE_MIN_TF = 1
+REDUCE_MAX_TF = 2
+REDUCE_MASS = 3
+REDUCE_TF_SUM = 4
+REDUCE_TS_SUM = 5
+REDUCE_FIX_SUM = 6
+REDUCE_SPECIES_SUM = 7
+
+KERNEL_BULK, KERNEL_NEE, KERNEL_FACE, KERNEL_STEP = 0, 1, 2, 3
+
+_c_int = ctypes.c_int
+_c_double = ctypes.c_double
+
+
+class LbmDesc(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", _c_int), ("dim", _c_int), ("nx", _c_int), ("ny", _c_int), ("nz", _c_int),
+        ("thermal", _c_int), ("chemical", _c_int), ("poro", _c_int), ("radiation", _c_int),
+        ("n_species", _c_int), ("species_fix", _c_int * MAX_SPECIES), ("precision", _c_int),
+        ("collision", _c_int), ("device", _c_int),
+    ]
+
+
+class LbmThermalParams(ctypes.Structure):
+    _fields_ = [
+        ("v_ref", _c_double), ("v_scale", _c_double), ("diff_model", _c_int), ("diff", _c_double),
+        ("Pr", _c_double), ("cond_model", _c_int), ("cond", _c_double), ("cond_poly", _c_double * 5),
+        ("cap_model", _c_int), ("cm", _c_double), ("cm_poly", _c_double * 5), ("Trange", _c_double * 3),
+        ("nasa", (_c_double * 7) * 2), ("bc_type", _c_int * MAX_FACES), ("bc_value", _c_double * MAX_FACES),
+    ]
+
+
+class LbmSpeciesParams(ctypes.Structure):
+    _fields_ = [
+        ("molemass", _c_double), ("visc_model", _c_int), ("visco", _c_double), ("sutherland", _c_double * 2),
+        ("diff_model", _c_int), ("diff", _c_double), ("Sc", _c_double), ("thermo_model", _c_int),
+        ("enthalpy", _c_double), ("capa", _c_double), ("cond_model", _c_int), ("cond", _c_double),
+        ("cond_poly", _c_double * 5), ("bc_type", _c_int * MAX_FACES), ("bc_value", _c_double * MAX_FACES),
+    ]
+
+
+class LbmReactionParams(ctypes.Structure):
+    _fields_ = [
+        ("A", _c_double), ("Ea", _c_double), ("b", _c_double), ("Tmin", _c_double),
+        ("coef_reactant", _c_double * MAX_SPECIES), ("coef_product", _c_double * MAX_SPECIES),
+        ("coef_rate", _c_double * MAX_SPECIES),
+    ]
+
+
+class LbmParams(ctypes.Structure):
+    _fields_ = [
+        ("dx", _c_double), ("dt", _c_double), ("tol", _c_double), ("eos", _c_int), ("visc_model", _c_int),
+        ("visco", _c_double), ("sutherland", _c_double * 2), ("force_term", _c_int), ("bc_model", _c_int),
+        ("poro_model", _c_int), ("bc_flow", _c_int * MAX_FACES), ("bc_v", _c_int * MAX_FACES),
+        ("bc_rho", _c_int * MAX_FACES), ("v_bc", (_c_double * 3) * MAX_FACES), ("rho_bc", _c_double * MAX_FACES),
+        ("flow_bc", _c_double * MAX_FACES), ("tf", LbmThermalParams), ("ts", LbmThermalParams),
+        ("radiation_model", _c_int), ("t_ambient", _c_double), ("n_reactions", _c_int),
+        ("species", LbmSpeciesParams * MAX_SPECIES), ("reactions", LbmReactionParams * MAX_REACTIONS),
+        ("mrt_rates", _c_double * 19), ("check_divergence", _c_int), ("max_velocity", _c_double),
+    ]
+
+
+class LbmError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{message} (status {code})")
+        self.code = code
+
+
+class DivergenceError(LbmError):
+    """The divergence detector tripped (NaN or |u| above the configured limit)."""
+
+
+_SYMBOLS = [
+    "lbm_abi_version", "lbm_struct_size", "lbm_create", "lbm_destroy", "lbm_set_params", "lbm_set_field",
+    "lbm_get_field", "lbm_init", "lbm_step", "lbm_reduce", "lbm_sync", "lbm_steps_done", "lbm_last_error",
+    "lbm_profile", "lbm_profile_read",
+]
+
+
+class CLib:
+    """Typed wrapper of one shared library that exports the lbm_* C ABI."""
+
+    def __init__(self, path):
+        self.path = os.path.abspath(path)
+        self.lib = ctypes.CDLL(self.path)
+        L = self.lib
+        L.lbm_abi_version.restype = _c_int
+        L.lbm_struct_size.argtypes = [_c_int]
+        L.lbm_struct_size.restype = ctypes.c_size_t
+        L.lbm_create.argtypes = [ctypes.POINTER(LbmDesc), ctypes.POINTER(ctypes.c_void_p)]
+        L.lbm_create.restype = _c_int
+        L.lbm_destroy.argtypes = [ctypes.c_void_p]
+        L.lbm_destroy.restype = None
+        L.lbm_set_params.argtypes = [ctypes.c_void_p, ctypes.POINTER(LbmParams)]
+        L.lbm_set_params.restype = _c_int
+        L.lbm_set_field.argtypes = [ctypes.c_void_p, _c_int, _c_int, ctypes.c_void_p, _c_int, ctypes.c_size_t]
+        L.lbm_set_field.restype = _c_int
+        L.lbm_get_field.argtypes = [ctypes.c_void_p, _c_int, _c_int, ctypes.c_void_p, _c_int, ctypes.c_size_t]
+        L.lbm_get_field.restype = _c_int
+        L.lbm_init.argtypes = [ctypes.c_void_p]
+        L.lbm_init.restype = _c_int
+        L.lbm_step.argtypes = [ctypes.c_void_p, _c_int]
+        L.lbm_step.restype = _c_int
+        L.lbm_reduce.argtypes = [ctypes.c_void_p, _c_int, _c_int, ctypes.POINTER(_c_double)]
+        L.lbm_reduce.restype = _c_int
+        L.lbm_sync.argtypes = [ctypes.c_void_p]
+        L.lbm_sync.restype = _c_int
+        L.lbm_steps_done.argtypes = [ctypes.c_void_p]
+        L.lbm_steps_done.restype = ctypes.c_longlong
+        L.lbm_last_error.argtypes = [ctypes.c_void_p]
+        L.lbm_last_error.restype = ctypes.c_char_p
+        L.lbm_profile.argtypes = [ctypes.c_void_p, _c_int]
+        L.lbm_profile.restype = _c_int
+        L.lbm_profile_read.argtypes = [ctypes.c_void_p, _c_int, ctypes.POINTER(_c_double),
+                                       ctypes.POINTER(ctypes.c_longlong)]
+        L.lbm_profile_read.restype = _c_int
+        if L.lbm_abi_version() != ABI_VERSION:
+            raise LbmError(ERR_INVALID, f"{self.path}: ABI version {L.lbm_abi_version()} != {ABI_VERSION}")
+        if L.lbm_struct_size(0) != ctypes.sizeof(LbmDesc) or L.lbm_struct_size(1) != ctypes.sizeof(LbmParams):
+            raise LbmError(ERR_INVALID, f"{self.path}: struct layout differs from the Python binding")
+
+    # -- helpers -----------------------------------------------------------
+    def _check(self, h, rc, what):
+        if rc != OK:
+            msg = self.lib.lbm_last_error(h).decode(errors="replace") if h else ""
+            cls = DivergenceError if rc == ERR_DIVERGED else LbmError
+            raise cls(rc, f"{what}: {msg or 'failed'}")
+
+    def create(self, desc):
+        h = ctypes.c_void_p()
+        rc = self.lib.lbm_create(ctypes.byref(desc), ctypes.byref(h))
+        if rc != OK:
+            names = {ERR_INVALID: "invalid descriptor", ERR_UNSUPPORTED: "unsupported configuration",
+                     ERR_OOM: "out of memory", ERR_DEVICE: "device error"}
+            raise LbmError(rc, f"lbm_create: {names.get(rc, 'failed')}")
+        return h
+
+    def destroy(self, h):
+        if h:
+            self.lib.lbm_destroy(h)
+
+    def set_params(self, h, params):
+        self._check(h, self.lib.lbm_set_params(h, ctypes.byref(params)), "lbm_set_params")
+
+    def set_field(self, h, field, comp, array):
+        a = np.ascontiguousarray(array, dtype=np.float64).ravel()
+        self._check(h, self.lib.lbm_set_field(h, field, comp, a.ctypes.data, DTYPE_F64, a.size), "lbm_set_field")
+
+    def get_field(self, h, field, comp, n):
+        out = np.empty(n, dtype=np.float64)
+        self._check(h, self.lib.lbm_get_field(h, field, comp, out.ctypes.data, DTYPE_F64, n), "lbm_get_field")
+        return out
+
+    def init(self, h):
+        self._check(h, self.lib.lbm_init(h), "lbm_init")
+
+    def step(self, h, n):
+        self._check(h, self.lib.lbm_step(h, int(n)), "lbm_step")
+
+    def reduce(self, h, kind, comp=0):
+        out = _c_double()
+        self._check(h, self.lib.lbm_reduce(h, kind, comp, ctypes.byref(out)), "lbm_reduce")
+        return out.value
+
+    def sync(self, h):
+        self._check(h, self.lib.lbm_sync(h), "lbm_sync")
+
+    def profile(self, h, enable):
+        self._check(h, self.lib.lbm_profile(h, int(enable)), "lbm_profile")
+
+    def profile_read(self, h, kernel):
+        ms = _c_double()
+        n = ctypes.c_longlong()
+        self._check(h, self.lib.lbm_profile_read(h, kernel, ctypes.byref(ms), ctypes.byref(n)), "lbm_profile_read")
+        return ms.value, n.value
+
+    def steps_done(self, h):
+        return int(self.lib.lbm_steps_done(h))
+
+
+_PRODUCT = None
+
+
+def library_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblbm_mi355x.so")
+
+
+def load_library():
+    """Open the HIP library (built by __graft_entry__.build() / lbm_porouschem_amd/build.py)."""
+    global _PRODUCT
+    if _PRODUCT is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is missing: build the gfx950 library first "
+                "(python -c 'import __graft_entry__ as g; g.build()' or python -m lbm_porouschem_amd.build)")
+        _PRODUCT = CLib(path)
+    return _PRODUCT

@@ -1,0 +1,1306 @@
+// lbm_core.hpp -- per-cell logic of the coupled lattice-Boltzmann step for the
+// gfx950 kernels in lbm_kernels.hip.
+//
+// Storage model (all arrays structure-of-arrays, cell index c = (x*ny + y)*nz + z):
+//   pops[slot][N]   one array per lattice direction ("slot"), A-A in-place:
+//                   flow slots 0..Q-1, then TF, TS, then the fluid species,
+//                   QS slots each.  Even steps read and write a cell's own slots
+//                   (post-collision value of direction k goes to slot LR[k]); odd
+//                   steps read slot LR[k] of the upstream neighbour and write slot
+//                   k of the downstream neighbour, so every population is read once
+//                   and written once per step and one copy of the lattice suffices.
+//   Cells that a lattice does not evolve (flow/TF/species where solid >= 1, TS
+//   where solid == 0) keep their populations in fixed slots: slot k = direction k.
+//   mac[...]        the reference's macroscopic fields (rho, u, TF.S, TS.S, species
+//                   S/dS, rhos).  Always current on the boundary shell (cells at
+//                   distance <= 1 from a face, which the NEE boundary reads and
+//                   writes) and refreshed everywhere on steps that request it.
+//   side[slot][Nsh] incoming populations of the shell cells (the NEE's f_nb / G_nb).
+//   fixS[NX][N]     FIX-species densities one collision ahead (S + dS applied).
+//   lag[N]          temperature read by the Arrhenius rate of the next macro
+//                   (TS.S where solid > 0, else TF.S): the reference evaluates the
+//                   reaction with the previous step's temperature
+//                   (LBM3D/_evolution.py:174 runs before :180-189).
+//
+// One time step n of the reference (LBM3D/_evolution.py:16-25: collide+push ->
+// macro -> NEE) maps onto the kernels as
+//   bulk<parity>      gather = push result of collision n; macro n; interior cells
+//                     then perform collision n+1 and scatter it.
+//   nee_x, nee_y[, nee_z]  NEE of face cells in the reference's face order.
+//   face_collide<parity>   collision n+1 of the face cells from post-BC state.
+// so the stored populations are always "one collision ahead" of the reference's f.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#ifndef LBM_DEV
+#define LBM_DEV __device__ __forceinline__
+#endif
+#ifndef LBM_HD
+#define LBM_HD __host__ __device__ __forceinline__
+#endif
+
+namespace lbm {
+
+constexpr int MAXS = 16;   // species (matches LBM_MAX_SPECIES)
+constexpr int MAXR = 16;   // reactions
+constexpr double GAS_R = 8.314441;        // util/constant.py:17
+constexpr double SIGMA_SB = 5.670374419e-8;  // util/constant.py:19
+
+// enum values (src/LBM/util/flag.py)
+enum { VISC_NONE = 0, VISC_CONSTANT = 1, VISC_SUTHERLAND = 2, VISC_MIXTURE = 3 };
+enum { PORO_SPHERICAL = 0, PORO_DARCY = 1, PORO_DF = 2, PORO_DARCY_HIGH = 3 };
+enum { COND_NONE = 0, COND_CONSTANT = 1, COND_POLY = 2, COND_MIXTURE = 3 };
+enum { THERMO_NONE = 0, THERMO_CONSTANT = 1, THERMO_POLY = 2, THERMO_NASA7 = 3, THERMO_MIXTURE = 4 };
+enum { TDIFF_NONE = 0, TDIFF_CONSTANT = 1, TDIFF_DERIVED = 2, TDIFF_PRANDTL = 3 };
+enum { DIFF_NONE = 0, DIFF_CONSTANT = 1, DIFF_SCHMIDT = 2 };
+enum { EOS_IDEAL = 0, EOS_INCOMP = 1 };
+enum { BC_PERIODIC = 0, BC_ZEROGRAD = 1, BC_FIXED = 2 };
+
+// mask bits per cell: bit k (k < Q): flow lattice evolves at neighbour c+e_k (solid < 1);
+// bit TS_BIT+k (k < QS): TS evolves at neighbour c+e_k (solid > 0).  Bit 0 / TS_BIT are the cell itself.
+constexpr int TS_BIT = 19;
+constexpr uint32_t FACE_BIT = 1u << 26;   // distance 0 from a face
+constexpr uint32_t SHELL_BIT = 1u << 27;  // distance <= 1 from a face
+constexpr uint32_t SOLIDNZ_BIT = 1u << 28; // solid fraction != 0 (the kernel must read it)
+
+// ---------------------------------------------------------------------------
+// velocity sets: LBM3D/_initialize.py:25-43, LBM2D/_initialize.py:24-35
+// ---------------------------------------------------------------------------
+template <int D> struct Lat;
+template <> struct Lat<3> {
+    static constexpr int Q = 19, QS = 7;
+    LBM_DEV static constexpr int ex(int k) {
+        return k == 1 || k == 7 || k == 9 || k == 11 || k == 13 ? 1 : (k == 2 || k == 8 || k == 10 || k == 12 || k == 14 ? -1 : 0);
+    }
+    LBM_DEV static constexpr int ey(int k) {
+        return k == 3 || k == 7 || k == 8 || k == 15 || k == 17 ? 1 : (k == 4 || k == 9 || k == 10 || k == 16 || k == 18 ? -1 : 0);
+    }
+    LBM_DEV static constexpr int ez(int k) {
+        return k == 5 || k == 11 || k == 12 || k == 15 || k == 16 ? 1 : (k == 6 || k == 13 || k == 14 || k == 17 || k == 18 ? -1 : 0);
+    }
+    // opposite direction (LBM3D/_core.py:36)
+    LBM_DEV static constexpr int lr(int k) {
+        return k == 0 ? 0 : (k <= 6 ? ((k & 1) ? k + 1 : k - 1) : (k <= 10 ? 17 - k : (k <= 14 ? 25 - k : 33 - k)));
+    }
+    LBM_DEV static constexpr double w(int k) { return k == 0 ? 1.0 / 3.0 : (k < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
+    LBM_DEV static constexpr double ws(int k) { return k == 0 ? 1.0 / 4.0 : 1.0 / 8.0; }
+};
+template <> struct Lat<2> {
+    static constexpr int Q = 9, QS = 5;
+    // ordering 0,E,N,W,S,NE,NW,SW,SE
+    LBM_DEV static constexpr int ex(int k) { return k == 1 || k == 5 || k == 8 ? 1 : (k == 3 || k == 6 || k == 7 ? -1 : 0); }
+    LBM_DEV static constexpr int ey(int k) { return k == 2 || k == 5 || k == 6 ? 1 : (k == 4 || k == 7 || k == 8 ? -1 : 0); }
+    LBM_DEV static constexpr int ez(int) { return 0; }
+    LBM_DEV static constexpr int lr(int k) { return k == 0 ? 0 : (k <= 4 ? ((k + 1) & 3) + 1 : ((k - 3) & 3) + 5); }
+    LBM_DEV static constexpr double w(int k) { return k == 0 ? 4.0 / 9.0 : (k < 5 ? 1.0 / 9.0 : 1.0 / 36.0); }
+    LBM_DEV static constexpr double ws(int k) { return k == 0 ? 1.0 / 3.0 : 1.0 / 6.0; }
+};
+
+// ---------------------------------------------------------------------------
+// parameters as the kernels see them (built on the host from lbm_params)
+// ---------------------------------------------------------------------------
+template <class Real> struct ThermP {
+    int diff_model, cond_model, cap_model;
+    int bc_type[6];
+    Real v_ref, v_scale, diff_lu, Pr, cond, cm, T1;
+    Real cond_poly[5], cm_poly[5], nasa[2][5];
+    Real bc_value[6];
+};
+template <class Real> struct SpecP {  // device species order: fluid species first, then FIX
+    int visc_model, diff_model, cond_model;
+    int bc_type[6];
+    Real molemass, visc_lu, suth0, suth1, diff_lu, Sc, enthalpy, cap_m, cond;
+    Real cond_poly[5];
+    Real bc_value[6];
+};
+template <class Real> struct ReacP {
+    Real A, Ea, b, Tmin;
+    Real reac[MAXS], prod[MAXS], rate[MAXS];  // device species order
+};
+
+template <class Real> struct DevParams {
+    int dim, nx, ny, nz;
+    long long N, nshell, nface;
+    int thermal, chem, poro, radiation;
+    int nf, nfix;             // species counts in use (<= template maxima)
+    int eos, visc_model, force_guo, poro_model, rad_model, bc_nee;
+    int bc_v[6], bc_rho[6];
+    int lag_tf, lag_sp;       // extra lagged fields needed by the porous correction
+    int n_reac;
+    int slot_tf, slot_ts, slot_sp, nslots;
+    Real tol, dt, dtdx2, dx, visc_lu, suth0, suth1, t_amb4, rho_bc[6];
+    ThermP<Real> tf, ts;
+    SpecP<Real> sp[MAXS];
+    ReacP<Real> r[MAXR];
+};
+
+// device buffers (pointers only; passed by value to every kernel)
+template <class Real, class Store> struct Bufs {
+    Store* pops;
+    Real* side;           // [nslots][nshell]
+    const uint32_t* mask;
+    const Real* solid;
+    const Real* cd;       // coefDarcy
+    const Real* cf;       // coefForchheimer
+    const Real* ha;       // exchangeCoef * exchangeSurface
+    const Real* rad;      // emissivity * radiation_surface (premultiplied)
+    const Real* rhos_static;  // solid density when it is not recomputed from FIX species
+    Real* fixS;           // [nfix][N]
+    Real* lag;            // [N]
+    Real* lagTF;          // [N] (optional)
+    Real* lagS;           // [nf][N] (optional)
+    Real* emit;           // [QS][N] TS populations of fully solid cells (optional)
+    // macroscopic fields (the reference's Taichi fields)
+    Real* rho;
+    Real* u;              // [3][N]
+    Real* TF;
+    Real* TS;
+    Real* dTF;
+    Real* dTS;
+    Real* S;              // [nf+nfix][N]
+    Real* dS;             // [nf+nfix][N]
+    Real* rhos;
+    Real* vprof;          // face velocity profiles, 3 values per face cell, faces concatenated
+    Real* rprof;          // face density profiles
+    long long prof_off[6];
+};
+
+// ---------------------------------------------------------------------------
+// population storage codecs
+// ---------------------------------------------------------------------------
+template <class Store, class Real> struct Codec {
+    LBM_DEV static Real load(const Store* p, Real) { return (Real)*p; }
+    LBM_DEV static void store(Store* p, Real v, Real) { *p = (Store)v; }
+};
+
+// ---------------------------------------------------------------------------
+// geometry helpers
+// ---------------------------------------------------------------------------
+LBM_DEV void cell_coord(long long c, int ny, int nz, int& x, int& y, int& z) {
+    z = (int)(c % nz);
+    long long r = c / nz;
+    y = (int)(r % ny);
+    x = (int)(r / ny);
+}
+
+// Index of a cell inside the band of cells at distance < d from any face
+// (d = 2: boundary shell, d = 1: face cells).  Returns -1 outside the band.
+template <int D>
+LBM_HD long long band_index(int x, int y, int z, int nx, int ny, int nz, int d) {
+    const int dz = D == 3 ? d : 0;
+    long long A = 2LL * dz * nx * ny;
+    if (D == 3 && (z < dz || z >= nz - dz)) {
+        int zr = z < dz ? z : z - (nz - 2 * dz);
+        return ((long long)zr * nx + x) * ny + y;
+    }
+    long long B = 2LL * d * nx * (nz - 2 * dz);
+    if (y < d || y >= ny - d) {
+        int yr = y < d ? y : y - (ny - 2 * d);
+        return A + ((long long)(z - dz) * 2 * d + yr) * nx + x;
+    }
+    if (x < d || x >= nx - d) {
+        int xr = x < d ? x : x - (nx - 2 * d);
+        return A + B + ((long long)(z - dz) * (ny - 2 * d) + (y - d)) * 2 * d + xr;
+    }
+    return -1;
+}
+template <int D>
+LBM_HD long long band_count(int nx, int ny, int nz, int d) {
+    const int dz = D == 3 ? d : 0;
+    return 2LL * dz * nx * ny + 2LL * d * nx * (nz - 2 * dz) + 2LL * d * (ny - 2 * d) * (nz - 2 * dz);
+}
+template <int D>
+LBM_HD void band_coord(long long i, int nx, int ny, int nz, int d, int& x, int& y, int& z) {
+    const int dz = D == 3 ? d : 0;
+    long long A = 2LL * dz * nx * ny;
+    long long B = 2LL * d * nx * (nz - 2 * dz);
+    if (i < A) {
+        y = (int)(i % ny);
+        long long r = i / ny;
+        x = (int)(r % nx);
+        int zr = (int)(r / nx);
+        z = zr < dz ? zr : zr + (nz - 2 * dz);
+        return;
+    }
+    i -= A;
+    if (i < B) {
+        x = (int)(i % nx);
+        long long r = i / nx;
+        int yr = (int)(r % (2 * d));
+        z = (int)(r / (2 * d)) + dz;
+        y = yr < d ? yr : yr + (ny - 2 * d);
+        return;
+    }
+    i -= B;
+    int xr = (int)(i % (2 * d));
+    long long r = i / (2 * d);
+    y = (int)(r % (ny - 2 * d)) + d;
+    z = (int)(r / (ny - 2 * d)) + dz;
+    x = xr < d ? xr : xr + (nx - 2 * d);
+}
+
+// neighbour c + e (periodic wrap as LBM3D/_evolution.py:201-210)
+template <int D>
+LBM_DEV long long nbr(long long c, int x, int y, int z, int ex, int ey, int ez, int nx, int ny, int nz, bool edge) {
+    if (!edge) return c + ((long long)ex * ny + ey) * nz + ez;
+    int a = x + ex, b = y + ey, cc = z + ez;
+    a = a < 0 ? nx - 1 : (a > nx - 1 ? 0 : a);
+    b = b < 0 ? ny - 1 : (b > ny - 1 ? 0 : b);
+    if (D == 3) cc = cc < 0 ? nz - 1 : (cc > nz - 1 ? 0 : cc);
+    return ((long long)a * ny + b) * nz + cc;
+}
+
+// ---------------------------------------------------------------------------
+// property models (LBM3D/_evolution.py:227-248, _thermal.py, _chemical.py)
+// ---------------------------------------------------------------------------
+template <class Real> LBM_DEV Real phys(const ThermP<Real>& t, Real v) { return v * t.v_scale + t.v_ref; }
+
+template <class Real> LBM_DEV Real poly5_ref(const Real* c, Real T) {
+    return c[0] + T * (c[1] + T * (c[2] * T + T * (c[3] + T * c[4])));
+}
+
+// Specie.viscosity (LBM3D/_chemical.py:70-77); TFlu = TF.S in lattice units
+template <class Real> LBM_DEV Real sp_visc(const DevParams<Real>& P, int j, Real TFlu) {
+    const SpecP<Real>& s = P.sp[j];
+    Real visco = (Real)1e-5;
+    if (s.visc_model == VISC_CONSTANT) {
+        visco = s.visc_lu;
+    } else if (s.visc_model == VISC_SUTHERLAND) {
+        Real T = P.thermal ? phys(P.tf, TFlu) : (Real)273.15;
+        visco = s.suth0 * (Real)pow(T, (Real)1.5) / (T + s.suth1);
+    }
+    return visco;
+}
+
+// viscosity(i) with the TF.S / species S values the caller decides on
+template <class Real, int NF>
+LBM_DEV Real visc_dyn(const DevParams<Real>& P, Real TFlu, const Real* Sfl) {
+    Real visco = (Real)0.1;
+    if (P.visc_model == VISC_CONSTANT) {
+        visco = P.visc_lu;
+    } else if (P.visc_model == VISC_SUTHERLAND) {
+        Real T = P.thermal ? phys(P.tf, TFlu) : (Real)273.15;
+        visco = P.suth0 * (Real)pow(T, (Real)1.5) / (T + P.suth1) * P.dtdx2;
+    } else if (P.visc_model == VISC_MIXTURE) {
+        if (P.chem) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+                if (j < P.nf) visco += Sfl[j] * sp_visc(P, j, TFlu);
+        }
+    }
+    return visco;
+}
+template <class Real> LBM_DEV Real kin_from_dyn(const DevParams<Real>& P, Real visc, Real rho) {
+    return P.eos == EOS_IDEAL ? visc / rho : visc;
+}
+
+template <class Real> LBM_DEV Real sp_cond(const DevParams<Real>& P, int j, Real TFlu) {
+    const SpecP<Real>& s = P.sp[j];
+    Real cond = (Real)0.0;
+    if (s.cond_model == COND_CONSTANT) {
+        cond += s.cond;
+    } else if (s.cond_model == COND_POLY) {
+        Real T = P.thermal ? phys(P.tf, TFlu) : (Real)273.15;
+        const Real* c = s.cond_poly;
+        cond += (((c[4] * T + c[3]) * T + c[2]) * T + c[1]) * T + c[0];
+    }
+    return cond;
+}
+
+template <class Real, int NF>
+LBM_DEV Real tf_cond(const DevParams<Real>& P, Real TFlu, const Real* S) {
+    const ThermP<Real>& t = P.tf;
+    Real kappa = (Real)0.0;
+    if (t.cond_model == COND_CONSTANT) kappa = t.cond;
+    else if (t.cond_model == COND_POLY) kappa = poly5_ref(t.cond_poly, phys(t, TFlu));
+    else if (t.cond_model == COND_MIXTURE) {
+        if (P.chem) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+                if (j < P.nf) kappa += sp_cond(P, j, TFlu) * S[j];
+        }
+    }
+    return kappa;
+}
+template <class Real, int NF>
+LBM_DEV Real tf_cap(const DevParams<Real>& P, Real TFlu, const Real* S) {
+    const ThermP<Real>& t = P.tf;
+    Real cm = (Real)0.0;
+    if (t.cap_model == THERMO_CONSTANT) cm = t.cm;
+    else if (t.cap_model == THERMO_POLY) cm = poly5_ref(t.cm_poly, phys(t, TFlu));
+    else if (t.cap_model == THERMO_NASA7) {
+        Real T = phys(t, TFlu);
+        cm = T < t.T1 ? poly5_ref(t.nasa[0], T) : poly5_ref(t.nasa[1], T);
+        cm *= (Real)GAS_R;
+    } else if (t.cap_model == THERMO_MIXTURE) {
+        if (P.chem) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+                if (j < P.nf) cm += P.sp[j].cap_m * S[j];
+        }
+    }
+    return cm;
+}
+// TemperatureSolid.conductivity: MIXTURE sums FIX species S_j/rhos
+template <class Real, int NF, int NX>
+LBM_DEV Real ts_cond(const DevParams<Real>& P, Real TSlu, Real TFlu, const Real* S, Real rhos) {
+    const ThermP<Real>& t = P.ts;
+    Real kappa = (Real)0.0;
+    if (t.cond_model == COND_CONSTANT) kappa = t.cond;
+    else if (t.cond_model == COND_POLY) {
+        Real T = phys(t, TSlu);
+        const Real* c = t.cond_poly;
+        kappa = c[0] + T * (c[1] + T * (c[2] * T + T * (c[3])));
+    } else if (t.cond_model == COND_MIXTURE) {
+        if (P.chem) {
+#pragma unroll
+            for (int j = 0; j < NX; ++j)
+                if (j < P.nfix) kappa += sp_cond(P, NF + j, TFlu) * S[NF + j] / rhos;
+        }
+    }
+    return kappa;
+}
+// TemperatureSolid.capacity_m: NASA7 without *R; MIXTURE sums fluid species * rho (D18)
+template <class Real, int NF>
+LBM_DEV Real ts_cap(const DevParams<Real>& P, Real TSlu, const Real* S, Real rho) {
+    const ThermP<Real>& t = P.ts;
+    Real cm = (Real)0.0;
+    if (t.cap_model == THERMO_CONSTANT) cm = t.cm;
+    else if (t.cap_model == THERMO_POLY) cm = poly5_ref(t.cm_poly, phys(t, TSlu));
+    else if (t.cap_model == THERMO_NASA7) {
+        Real T = phys(t, TSlu);
+        cm = T < t.T1 ? poly5_ref(t.nasa[0], T) : poly5_ref(t.nasa[1], T);
+    } else if (t.cap_model == THERMO_MIXTURE) {
+        if (P.chem) {
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+                if (j < P.nf) cm += P.sp[j].cap_m * S[j] * rho;
+        } else {
+            cm += 1000;
+        }
+    }
+    return cm;
+}
+
+template <int D, class Real> LBM_DEV Real scalar_tau(Real Dlu) {
+    return D == 3 ? 4 * Dlu + (Real)0.5 : 3 * Dlu + (Real)0.5;
+}
+
+// ---------------------------------------------------------------------------
+// equilibria
+// ---------------------------------------------------------------------------
+// feq19 / feq9 (LBM3D/_evolution.py:213-225)
+template <int D, class Real>
+LBM_DEV Real feq(int eos, int k, Real rho, Real ux, Real uy, Real uz, Real eps) {
+    using L = Lat<D>;
+    Real eu = (Real)L::ex(k) * ux + (Real)L::ey(k) * uy + (Real)L::ez(k) * uz;
+    Real uv = ux * ux + uy * uy + uz * uz;
+    Real w = (Real)L::w(k);
+    Real ee = eps + (Real)1e-12;
+    if (eos == EOS_INCOMP)
+        return w * (rho + (Real)3.0 * eu + (Real)4.5 * eu * eu / ee - (Real)1.5 * uv / ee);
+    return w * rho * ((Real)1.0 + (Real)3.0 * eu + (Real)4.5 * eu * eu / ee - (Real)1.5 * uv / ee);
+}
+// TF.geq (3D: 1+4eu+8eu^2-2u^2, w 1/4,1/8; 2D: 1+3eu+4.5eu^2-1.5u^2, w 1/3,1/6)
+template <int D, class Real>
+LBM_DEV Real geq_tf(int k, Real T, Real ux, Real uy, Real uz) {
+    using L = Lat<D>;
+    Real eu = (Real)L::ex(k) * ux + (Real)L::ey(k) * uy + (Real)L::ez(k) * uz;
+    Real uv = ux * ux + uy * uy + uz * uz;
+    Real w = (Real)L::ws(k);
+    if (D == 3) return w * T * ((Real)1.0 + (Real)4.0 * eu + (Real)8.0 * eu * eu - (Real)2.0 * uv);
+    return w * T * ((Real)1.0 + (Real)3.0 * eu + (Real)4.5 * eu * eu - (Real)1.5 * uv);
+}
+// species equilibrium: 3D quadratic (LBM3D/_chemical.py:131-137), 2D linear (LBM2D/_chemical.py:130-135)
+template <int D, class Real>
+LBM_DEV Real geq_sp(int k, Real S, Real ux, Real uy, Real uz) {
+    using L = Lat<D>;
+    Real eu = (Real)L::ex(k) * ux + (Real)L::ey(k) * uy + (Real)L::ez(k) * uz;
+    Real w = (Real)L::ws(k);
+    if (D == 3) {
+        Real uv = ux * ux + uy * uy + uz * uz;
+        return w * S * (1 + (Real)4.0 * eu + (Real)8.0 * eu * eu - 2 * uv);
+    }
+    return w * S * (1 + 3 * eu);
+}
+
+// ---------------------------------------------------------------------------
+// the per-cell state
+// ---------------------------------------------------------------------------
+template <class Real, int D, int NF, int NX> struct CellPops {
+    static constexpr int Q = Lat<D>::Q, QS = Lat<D>::QS;
+    Real f[Q];
+    Real gT[QS];       // TF
+    Real gS[QS];       // TS
+    Real gP[NF > 0 ? NF : 1][QS];  // fluid species
+};
+
+template <class Real, int NF, int NX> struct CellMac {
+    static constexpr int NS = NF + NX;
+    Real rho, ux, uy, uz;
+    Real sol, eps;
+    Real TF, TS;       // lattice units
+    Real S[NS > 0 ? NS : 1];   // fluid: normalised mass fraction; FIX: density
+    Real rhos;
+    Real dTF, dTS;
+    Real dS[NS > 0 ? NS : 1];
+    // static per-cell inputs
+    Real cd_, cf_;     // coefDarcy, coefForchheimer
+    Real ha_;          // exchangeCoef * exchangeSurface
+    Real rad_;         // emissivity * SIGMA * radiation_surface / dx
+};
+
+// ---------------------------------------------------------------------------
+// reactions (LBM3D/_chemical.py:189-270), lagged temperature Tlag (lattice units)
+// ---------------------------------------------------------------------------
+template <class Real, int NF, int NX>
+LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real Tlag, Real TFold) {
+    constexpr int NS = NF + NX;
+    Real acc[NS > 0 ? NS : 1];
+    Real accH = (Real)0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) acc[j] = (Real)0.0;
+    const bool solidT = m.sol > 0;
+    for (int r = 0; r < P.n_reac; ++r) {
+        const ReacP<Real>& R = P.r[r];
+        Real kr = (Real)0.0;
+        if (P.thermal) {
+            Real T = solidT ? phys(P.ts, Tlag) : phys(P.tf, Tlag);
+            if (T > R.Tmin) kr = R.A * (Real)pow(T + (Real)1e-6, R.b) * (Real)exp(-R.Ea / (T + (Real)1e-6) / (Real)GAS_R);
+        } else {
+            kr = R.A;
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const bool live = j < NF ? j < P.nf : (j - NF) < P.nfix;
+            if (live && R.reac[j] > 0) {
+                if (m.S[j] > P.tol) {
+                    if (R.rate[j] != 0) {
+                        Real base = j < NF ? m.S[j] * m.rho / P.sp[j].molemass : m.S[j] / P.sp[j].molemass;
+                        kr *= R.rate[j] == (Real)1 ? base : (Real)pow(base, R.rate[j]);
+                    }
+                } else {
+                    kr = 0;
+                }
+            }
+        }
+        Real dH = (Real)0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const bool live = j < NF ? j < P.nf : (j - NF) < P.nfix;
+            Real coef = -R.reac[j] + R.prod[j];
+            if (live && coef != (Real)0.) {
+                acc[j] += kr * coef * P.dt * P.sp[j].molemass;
+                dH += coef * P.sp[j].enthalpy;
+            }
+        }
+        if (P.thermal) accH += dH * (-kr * P.dt);
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) m.dS[j] = acc[j];
+    if (P.thermal) {
+        // capacity with the previous step's temperature (update_dS runs before the T update)
+        if (m.sol > 0) m.dTS += accH / ts_cap<Real, NF>(P, Tlag, m.S, m.rho) / m.rhos;
+        else m.dTF += accH / tf_cap<Real, NF>(P, TFold, m.S) / m.rho;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// macro (LBM3D/_evolution.py:118-196) from gathered populations.
+// Tlag: lagged reaction temperature; TFold: TF.S of the previous step (porous
+// viscosity models); Sold: previous species S (mixture viscosity).
+// ---------------------------------------------------------------------------
+template <class Real, int D, int NF, int NX>
+LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m,
+                   Real Tlag, Real TFold, const Real* Sold) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q, QS = L::QS;
+    Real rho = (Real)0.0, ux = (Real)0.0, uy = (Real)0.0, uz = (Real)0.0;
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+        Real F = p.f[s];
+        if (L::ex(s) == 1) ux += F; else if (L::ex(s) == -1) ux += -F;
+        if (L::ey(s) == 1) uy += F; else if (L::ey(s) == -1) uy += -F;
+        if (L::ez(s) == 1) uz += F; else if (L::ez(s) == -1) uz += -F;
+        rho += F;
+    }
+    if (P.eos == EOS_IDEAL) {
+        ux /= rho;
+        uy /= rho;
+        uz /= rho;
+    }
+    const Real eps = m.eps;
+    if (P.poro && eps != 0 && eps != 1) {
+        Real visc = visc_dyn<Real, NF>(P, TFold, Sold);
+        Real kin = kin_from_dyn(P, visc, rho);
+        Real c;
+        if (P.poro_model == PORO_SPHERICAL) {
+            Real a = (Real)1.000001 - eps;
+            Real perm = (Real)pow(eps, (Real)3) / (a * a) * 1;
+            Real Dc = (Real)1.0 / perm;
+            Real Fc = (Real)1.75 / (Real)sqrt(150 * (Real)pow(eps, (Real)3));
+            Real c0 = (Real)0.5 * (1 + eps * kin / 2 * Dc);
+            Real c1 = (Real)0.5 * eps * Fc;
+            Real vn = (Real)sqrt(ux * ux + uy * uy + uz * uz);
+            c = c0 + (Real)sqrt(c0 * c0 + c1 * vn);
+        } else if (P.poro_model == PORO_DARCY) {
+            c = 1 + eps * kin * m.cd_ / 2;
+        } else if (P.poro_model == PORO_DF) {
+            Real c0 = (Real)0.5 * (1 + eps * kin / 2 * m.cd_);
+            Real c1 = (Real)0.5 * eps * m.cf_;
+            Real vn = (Real)sqrt(ux * ux + uy * uy + uz * uz);
+            c = c0 + (Real)sqrt(c0 * c0 + c1 * vn);
+        } else {
+            c = (Real)1;
+        }
+        if (P.poro_model <= PORO_DF) {
+            ux /= c;
+            uy /= c;
+            uz /= c;
+        }
+    }
+    m.rho = rho;
+    m.ux = ux;
+    m.uy = uy;
+    m.uz = uz;
+    m.TF = (Real)0.0;
+    m.TS = (Real)0.0;
+    m.dTF = (Real)0.0;
+    m.dTS = (Real)0.0;
+    if (P.chem) {
+        Real rhos = (Real)0.0;
+        Real Yall = (Real)0.0;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            if (j < P.nf) {
+                Real S = (Real)0.0;
+#pragma unroll
+                for (int s = 0; s < QS; ++s) S += p.gP[j][s];
+                if (S < -P.tol) S = -P.tol;
+                m.S[j] = S;
+                Yall += S;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (j < P.nfix && P.poro) rhos += m.S[NF + j];
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (j < P.nf) m.S[j] /= Yall;
+        if (P.poro) m.rhos = rhos;
+        reactions<Real, NF, NX>(P, m, Tlag, Tlag);
+    }
+    if (P.thermal) {
+        Real Tf = (Real)0.0, Ts = (Real)0.0;
+#pragma unroll
+        for (int s = 0; s < QS; ++s) {
+            Tf += p.gT[s];
+            Ts += p.gS[s];
+        }
+        m.TF = Tf;
+        m.TS = Ts;
+        if (m.sol > 0) {
+            Real dH = m.ha_ * (phys(P.tf, Tf) - phys(P.ts, Ts)) * P.dt;
+            m.dTS += dH / ts_cap<Real, NF>(P, Ts, m.S, m.rho) / m.rhos / P.ts.v_scale;
+            m.dTF += -dH / tf_cap<Real, NF>(P, Tf, m.S) / m.rho / P.tf.v_scale;
+        }
+        if (P.radiation && P.rad_model == 1) {
+            Real T = phys(P.ts, Ts);
+            Real q = (Real)0.0;
+            q += m.rad_ * (P.t_amb4 - (Real)pow(T, (Real)4));
+            m.dTS += q * P.dt / ts_cap<Real, NF>(P, Ts, m.S, m.rho) / m.rhos / P.ts.v_scale;
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// diffusivities (lattice units) of the scalar lattices
+// ---------------------------------------------------------------------------
+template <class Real, int NF, int NX>
+LBM_DEV Real tf_D(const DevParams<Real>& P, const CellMac<Real, NF, NX>& m, Real kin) {
+    const ThermP<Real>& t = P.tf;
+    if (t.diff_model == TDIFF_CONSTANT) return t.diff_lu;
+    if (t.diff_model == TDIFF_PRANDTL) return kin / t.Pr;
+    if (t.diff_model == TDIFF_DERIVED)
+        return tf_cond<Real, NF>(P, m.TF, m.S) / m.rho / tf_cap<Real, NF>(P, m.TF, m.S) * P.dtdx2;
+    return (Real)0.0;
+}
+template <class Real, int NF, int NX>
+LBM_DEV Real ts_D(const DevParams<Real>& P, const CellMac<Real, NF, NX>& m) {
+    const ThermP<Real>& t = P.ts;
+    if (t.diff_model == TDIFF_CONSTANT) return t.diff_lu;
+    if (t.diff_model == TDIFF_DERIVED)
+        return ts_cond<Real, NF, NX>(P, m.TS, m.TF, m.S, m.rhos) / m.rhos / ts_cap<Real, NF>(P, m.TS, m.S, m.rho) * P.dtdx2;
+    return (Real)0.0;
+}
+template <class Real> LBM_DEV Real sp_D(const DevParams<Real>& P, int j, Real TFlu) {
+    const SpecP<Real>& s = P.sp[j];
+    Real diff = (Real)0.0;
+    if (s.diff_model == DIFF_CONSTANT) diff += s.diff_lu;
+    else if (s.diff_model == DIFF_SCHMIDT) diff += s.Sc * sp_visc(P, j, TFlu) * P.dtdx2;
+    return diff;
+}
+
+// ---------------------------------------------------------------------------
+// collision + sources of one cell (LBM3D/_evolution.py:31-66), in place on p.
+// FIX species densities are advanced by dS (m.S[NF+j] += m.dS[NF+j]).
+// ---------------------------------------------------------------------------
+template <class Real, int D, int NF, int NX>
+LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, bool pts) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q, QS = L::QS;
+    const Real ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho, eps = m.eps;
+    const Real visc = visc_dyn<Real, NF>(P, m.TF, m.S);
+    const Real kin = kin_from_dyn(P, visc, rho);
+    const Real tau = 3 * kin + (Real)0.5;
+    const Real itau = 1 / tau;
+    // Guo forcing of the gray-LBM Darcy drag (darcy_force / forceTermGuo, :263-293)
+    Real Fx = 0, Fy = 0, Fz = 0;
+    const bool guo = P.force_guo != 0;
+    if (guo && P.poro && m.sol != 0 && m.sol != 1) {
+        const Real nu = D == 3 ? visc : kin;  // 3D dynamic, 2D kinematic viscosity
+        const Real e = (Real)1.0 - m.sol;
+        Real c = 0;
+        bool any = true;
+        if (P.poro_model == PORO_SPHERICAL) {
+            Real a = (Real)1.000001 - e;
+            Real perm = (Real)pow(e, (Real)3) / (a * a) * 1;
+            Real beta = (Real)1.75 / (Real)sqrt((Real)150 * e * perm);
+            c = -nu / perm * e - beta * (Real)sqrt(ux * ux + uy * uy + uz * uz);
+        } else if (P.poro_model == PORO_DARCY || (D == 2 && P.poro_model == PORO_DARCY_HIGH)) {
+            c = -e * nu * m.cd_;
+        } else if (P.poro_model == PORO_DF) {
+            c = -e * nu * m.cd_ - e * m.cf_ * (Real)sqrt(ux * ux + uy * uy + uz * uz);
+        } else {
+            any = false;
+        }
+        if (any) {
+            Fx = c * ux;
+            Fy = c * uy;
+            Fz = c * uz;
+        }
+    }
+    const Real ee = (P.eos == EOS_INCOMP && D == 2) ? eps : eps + (Real)1e-12;
+    const Real gpre = P.eos == EOS_INCOMP ? ((Real)1.0 - (Real)1.0 / (Real)2.0 / tau)
+                                          : ((Real)1.0 - (Real)1.0 / (Real)2.0 / tau) * rho;
+    const Real uox = ux / ee, uoy = uy / ee, uoz = uz / ee;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Real fe = feq<D, Real>(P.eos, k, rho, ux, uy, uz, eps);
+        Real f = p.f[k] - itau * (p.f[k] - fe);
+        if (guo) {
+            const Real ex = (Real)L::ex(k), ey = (Real)L::ey(k), ez = (Real)L::ez(k);
+            const Real emF = (ex - uox) * Fx + (ey - uoy) * Fy + (ez - uoz) * Fz;
+            const Real eu = ex * ux + ey * uy + ez * uz;
+            const Real eF = ex * Fx + ey * Fy + ez * Fz;
+            f += gpre * (Real)L::w(k) * ((Real)3.0 * emF + (Real)9.0 * eu * eF / ee);
+        }
+        p.f[k] = f;
+    }
+    if (P.thermal) {
+        const Real itT = 1 / scalar_tau<D, Real>(tf_D<Real, NF, NX>(P, m, kin));
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            Real g = p.gT[k] - itT * (p.gT[k] - geq_tf<D, Real>(k, m.TF, ux, uy, uz));
+            p.gT[k] = g + geq_tf<D, Real>(k, m.dTF, ux, uy, uz);
+        }
+    }
+    // FIX species: S += dS at direction 0; with a mixture solid conductivity the
+    // TS relaxation time of directions >= 1 sees the updated densities.
+    Real itS0 = 0, itS1 = 0;
+    if (P.thermal && pts) itS0 = 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX>(P, m));
+    if (P.chem) {
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (j < P.nfix) m.S[NF + j] += m.dS[NF + j];
+    }
+    if (P.thermal && pts) {
+        itS1 = (P.chem && P.ts.diff_model == TDIFF_DERIVED && P.ts.cond_model == COND_MIXTURE)
+                   ? 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX>(P, m))
+                   : itS0;
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            const Real w = (Real)L::ws(k);
+            const Real it = k == 0 ? itS0 : itS1;
+            Real g = p.gS[k] - it * (p.gS[k] - w * m.TS);
+            p.gS[k] = g + w * m.dTS;
+        }
+    }
+    if (P.chem) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            if (j < P.nf) {
+                const Real it = 1 / scalar_tau<D, Real>(sp_D(P, j, m.TF));
+#pragma unroll
+                for (int k = 0; k < QS; ++k) {
+                    Real g = p.gP[j][k] - it * (p.gP[j][k] - geq_sp<D, Real>(k, m.S[j], ux, uy, uz));
+                    // 3D adds TF.geq7(dS) (same formula as the species equilibrium); 2D the linear geq5
+                    p.gP[j][k] = g + (D == 3 ? geq_tf<D, Real>(k, m.dS[j], ux, uy, uz) : geq_sp<D, Real>(k, m.dS[j], ux, uy, uz));
+                }
+            }
+        }
+    }
+}
+
+// TS of a fully solid cell: relax toward w*TS with no source (LBM3D/_evolution.py:98-104)
+template <class Real, int D, int NF, int NX>
+LBM_DEV void collide_emitter(const DevParams<Real>& P, Real* g, const CellMac<Real, NF, NX>& m) {
+    using L = Lat<D>;
+    const Real it = 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX>(P, m));
+#pragma unroll
+    for (int k = 0; k < L::QS; ++k) g[k] = g[k] - it * (g[k] - (Real)L::ws(k) * m.TS);
+}
+
+// ---------------------------------------------------------------------------
+// A-A population access
+// ---------------------------------------------------------------------------
+struct Geo {
+    long long c;
+    int x, y, z, nx, ny, nz;
+    bool edge;
+};
+template <int D> LBM_DEV long long nb_of(const Geo& g, int k) {
+    using L = Lat<D>;
+    return nbr<D>(g.c, g.x, g.y, g.z, L::ex(k), L::ey(k), L::ez(k), g.nx, g.ny, g.nz, g.edge);
+}
+
+// gather the incoming populations of one lattice (QN directions, slots base..base+QN-1)
+// bitbase: mask bit of direction 0 for this lattice's "evolves at neighbour" test
+template <int D, int QN, int PAR, class Real, class Store>
+LBM_DEV void gather(const Store* __restrict__ pops, long long N, int base, const Geo& g, uint32_t mask, int bitbase,
+                    Real* out) {
+    using L = Lat<D>;
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        long long src;
+        if (PAR == 0) {
+            src = (long long)(base + k) * N + g.c;
+        } else {
+            const int kr = L::lr(k);
+            if ((mask >> (bitbase + kr)) & 1u) src = (long long)(base + kr) * N + nb_of<D>(g, kr);
+            else src = (long long)(base + k) * N + g.c;
+        }
+        out[k] = Codec<Store, Real>::load(pops + src, (Real)0);
+    }
+}
+// scatter post-collision populations with the same parity rule
+template <int D, int QN, int PAR, class Real, class Store>
+LBM_DEV void scatter(Store* __restrict__ pops, long long N, int base, const Geo& g, uint32_t mask, int bitbase,
+                     const Real* in) {
+    using L = Lat<D>;
+#pragma unroll
+    for (int k = 0; k < QN; ++k) {
+        long long dst;
+        const int kr = L::lr(k);
+        if (PAR == 0) {
+            dst = (long long)(base + kr) * N + g.c;
+        } else {
+            if ((mask >> (bitbase + k)) & 1u) dst = (long long)(base + k) * N + nb_of<D>(g, k);
+            else dst = (long long)(base + kr) * N + g.c;
+        }
+        Codec<Store, Real>::store(pops + dst, in[k], (Real)0);
+    }
+}
+// fixed slots (lattice not evolved at this cell): slot k holds direction k
+template <int QN, class Real, class Store>
+LBM_DEV void load_fixed(const Store* __restrict__ pops, long long N, int base, long long c, Real* out) {
+#pragma unroll
+    for (int k = 0; k < QN; ++k) out[k] = Codec<Store, Real>::load(pops + (long long)(base + k) * N + c, (Real)0);
+}
+template <int QN, class Real, class Store>
+LBM_DEV void store_fixed(Store* __restrict__ pops, long long N, int base, long long c, const Real* in) {
+#pragma unroll
+    for (int k = 0; k < QN; ++k) Codec<Store, Real>::store(pops + (long long)(base + k) * N + c, in[k], (Real)0);
+}
+
+// side buffer (shell cells), same slot numbering as pops
+template <int QN, class Real>
+LBM_DEV void side_put(Real* __restrict__ side, long long ns, int base, long long si, const Real* in) {
+#pragma unroll
+    for (int k = 0; k < QN; ++k) side[(long long)(base + k) * ns + si] = in[k];
+}
+template <int QN, class Real>
+LBM_DEV void side_get(const Real* __restrict__ side, long long ns, int base, long long si, Real* out) {
+#pragma unroll
+    for (int k = 0; k < QN; ++k) out[k] = side[(long long)(base + k) * ns + si];
+}
+
+template <class Real, class Store, int D, int NF, int NX>
+LBM_DEV void side_put_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long si,
+                          const CellPops<Real, D, NF, NX>& p) {
+    using L = Lat<D>;
+    side_put<L::Q>(B.side, P.nshell, 0, si, p.f);
+    if (P.thermal) {
+        side_put<L::QS>(B.side, P.nshell, P.slot_tf, si, p.gT);
+        side_put<L::QS>(B.side, P.nshell, P.slot_ts, si, p.gS);
+    }
+#pragma unroll
+    for (int j = 0; j < NF; ++j)
+        if (P.chem && j < P.nf) side_put<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
+}
+template <class Real, class Store, int D, int NF, int NX>
+LBM_DEV void side_get_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long si,
+                          CellPops<Real, D, NF, NX>& p) {
+    using L = Lat<D>;
+    side_get<L::Q>(B.side, P.nshell, 0, si, p.f);
+    if (P.thermal) {
+        side_get<L::QS>(B.side, P.nshell, P.slot_tf, si, p.gT);
+        side_get<L::QS>(B.side, P.nshell, P.slot_ts, si, p.gS);
+    }
+#pragma unroll
+    for (int j = 0; j < NF; ++j)
+        if (P.chem && j < P.nf) side_get<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
+}
+
+// macroscopic fields
+template <class Real, class Store, int NF, int NX>
+LBM_DEV void mac_put(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, const CellMac<Real, NF, NX>& m) {
+    const long long N = P.N;
+    B.rho[c] = m.rho;
+    B.u[c] = m.ux;
+    B.u[N + c] = m.uy;
+    B.u[2 * N + c] = m.uz;
+    if (P.thermal) {
+        B.TF[c] = m.TF;
+        B.TS[c] = m.TS;
+        B.dTF[c] = m.dTF;
+        B.dTS[c] = m.dTS;
+        B.rhos[c] = m.rhos;
+    }
+    if (P.chem) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (j < P.nf) {
+                B.S[(long long)j * N + c] = m.S[j];
+                B.dS[(long long)j * N + c] = m.dS[j];
+            }
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (j < P.nfix) {
+                B.S[(long long)(P.nf + j) * N + c] = m.S[NF + j];
+                B.dS[(long long)(P.nf + j) * N + c] = m.dS[NF + j];
+            }
+        if (P.poro) B.rhos[c] = m.rhos;
+    }
+}
+template <class Real, class Store, int NF, int NX>
+LBM_DEV void mac_get(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, CellMac<Real, NF, NX>& m) {
+    const long long N = P.N;
+    m.rho = B.rho[c];
+    m.ux = B.u[c];
+    m.uy = B.u[N + c];
+    m.uz = B.u[2 * N + c];
+    m.TF = m.TS = m.dTF = m.dTS = (Real)0;
+    m.rhos = (Real)0;
+    if (P.thermal) {
+        m.TF = B.TF[c];
+        m.TS = B.TS[c];
+        m.dTF = B.dTF[c];
+        m.dTS = B.dTS[c];
+        m.rhos = B.rhos[c];
+    }
+#pragma unroll
+    for (int j = 0; j < NF + NX; ++j) {
+        m.S[j] = (Real)0;
+        m.dS[j] = (Real)0;
+    }
+    if (P.chem) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (j < P.nf) {
+                m.S[j] = B.S[(long long)j * N + c];
+                m.dS[j] = B.dS[(long long)j * N + c];
+            }
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (j < P.nfix) {
+                m.S[NF + j] = B.S[(long long)(P.nf + j) * N + c];
+                m.dS[NF + j] = B.dS[(long long)(P.nf + j) * N + c];
+            }
+        if (P.poro) m.rhos = B.rhos[c];
+    }
+}
+
+template <class Real, class Store, int NF, int NX>
+LBM_DEV void load_static(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, uint32_t mask,
+                         CellMac<Real, NF, NX>& m) {
+    m.sol = (mask & SOLIDNZ_BIT) ? B.solid[c] : (Real)0;
+    m.eps = 1 - m.sol;
+    m.cd_ = m.cf_ = m.ha_ = m.rad_ = (Real)0;
+    if (P.poro && m.sol != 0) {
+        if (P.poro_model == PORO_DARCY || P.poro_model == PORO_DF || P.poro_model == PORO_DARCY_HIGH) m.cd_ = B.cd[c];
+        if (P.poro_model == PORO_DF) m.cf_ = B.cf[c];
+    }
+    if (P.thermal && m.sol > 0) m.ha_ = B.ha[c];
+    if (P.thermal && P.radiation) m.rad_ = B.rad[c];
+}
+
+template <class Real, class Store, int NF, int NX>
+LBM_DEV void lag_put(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, const CellMac<Real, NF, NX>& m) {
+    if (P.chem && P.thermal) B.lag[c] = m.sol > 0 ? m.TS : m.TF;
+    if (P.lag_tf) B.lagTF[c] = m.TF;
+    if (P.lag_sp) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (j < P.nf) B.lagS[(long long)j * P.N + c] = m.S[j];
+    }
+}
+
+template <int D, int PAR, class Real, class Store, int NF, int NX>
+LBM_DEV void scatter_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, const Geo& g, uint32_t mask,
+                         const CellPops<Real, D, NF, NX>& p, bool pts) {
+    using L = Lat<D>;
+    scatter<D, L::Q, PAR>(B.pops, P.N, 0, g, mask, 0, p.f);
+    if (P.thermal) {
+        scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_tf, g, mask, 0, p.gT);
+        if (pts) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_ts, g, mask, TS_BIT, p.gS);
+        else store_fixed<L::QS>(B.pops, P.N, P.slot_ts, g.c, p.gS);
+    }
+#pragma unroll
+    for (int j = 0; j < NF; ++j)
+        if (P.chem && j < P.nf) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_sp + j * L::QS, g, mask, 0, p.gP[j]);
+}
+
+// ---------------------------------------------------------------------------
+// kernel bodies
+// ---------------------------------------------------------------------------
+// Bulk: gather (streaming of collision n) -> macro n -> [shell: record] ->
+// [interior: collision n+1 + scatter].
+template <class Real, class Store, int D, int NF, int NX, int PAR>
+LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, int write_macro) {
+    using L = Lat<D>;
+    constexpr int QS = L::QS;
+    const long long N = P.N;
+    const uint32_t mask = B.mask[c];
+    Geo g;
+    g.c = c;
+    g.nx = P.nx;
+    g.ny = P.ny;
+    g.nz = P.nz;
+    cell_coord(c, P.ny, P.nz, g.x, g.y, g.z);
+    g.edge = (mask & FACE_BIT) != 0;
+    const bool shell = (mask & SHELL_BIT) != 0;
+    const bool face = (mask & FACE_BIT) != 0;
+    const bool pflow = (mask & 1u) != 0;
+    const bool pts = ((mask >> TS_BIT) & 1u) != 0;
+    const long long si = shell ? band_index<D>(g.x, g.y, g.z, P.nx, P.ny, P.nz, 2) : -1;
+
+    CellPops<Real, D, NF, NX> p;
+    CellMac<Real, NF, NX> m;
+    if (!pflow) {
+        // fully solid cell: flow, TF and species are frozen; TS relaxes from its own populations
+        if (shell) {
+            load_fixed<L::Q>(B.pops, N, 0, c, p.f);
+            if (P.thermal) {
+                load_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+                gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+            }
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+                if (P.chem && j < P.nf) load_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+            side_put_all<Real, Store, D, NF, NX>(P, B, si, p);
+        }
+        if (face || !P.thermal) return;
+        mac_get<Real, Store, NF, NX>(P, B, c, m);
+        load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (P.chem && j < P.nfix) m.S[NF + j] = B.fixS[(long long)j * N + c];
+        Real ge[QS];
+#pragma unroll
+        for (int k = 0; k < QS; ++k) ge[k] = B.emit[(long long)k * N + c];
+        collide_emitter<Real, D, NF, NX>(P, ge, m);
+#pragma unroll
+        for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = ge[k];
+        scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, ge);
+        return;
+    }
+
+    load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+    gather<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, p.f);
+    if (P.thermal) {
+        gather<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
+        if (pts) gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+        else if (shell || write_macro) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+        else {
+#pragma unroll
+            for (int k = 0; k < QS; ++k) p.gS[k] = (Real)0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NF; ++j)
+        if (P.chem && j < P.nf) gather<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+#pragma unroll
+    for (int j = 0; j < NF + NX; ++j) {
+        m.S[j] = (Real)0;
+        m.dS[j] = (Real)0;
+    }
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        if (P.chem && j < P.nfix) m.S[NF + j] = B.fixS[(long long)j * N + c];
+    m.rhos = (Real)0;
+    if (P.thermal && !(P.chem && P.poro) && (m.sol > 0 || P.radiation || shell || write_macro)) m.rhos = B.rhos[c];
+    const Real Tlag = (P.chem && P.thermal) ? B.lag[c] : (Real)0;
+    const Real TFold = P.lag_tf ? B.lagTF[c] : Tlag;
+    Real Sold[NF > 0 ? NF : 1];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) Sold[j] = (P.lag_sp && j < P.nf) ? B.lagS[(long long)j * N + c] : (Real)0;
+    macro<Real, D, NF, NX>(P, p, m, Tlag, TFold, Sold);
+
+    if (shell) {
+        side_put_all<Real, Store, D, NF, NX>(P, B, si, p);
+        mac_put<Real, Store, NF, NX>(P, B, c, m);
+        if (face) return;
+    } else if (write_macro) {
+        mac_put<Real, Store, NF, NX>(P, B, c, m);
+    }
+    lag_put<Real, Store, NF, NX>(P, B, c, m);
+    collide<Real, D, NF, NX>(P, p, m, pts);
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
+    scatter_all<D, PAR, Real, Store, NF, NX>(P, B, g, mask, p, pts);
+}
+
+// NEE of one face cell b with inner neighbour nb (LBM3D/_boundary.py:211-282,
+// LBM3D/_scalarField.py:83-141).  Reads and writes the side buffer and the
+// macroscopic fields.
+template <class Real, class Store, int D, int NF, int NX>
+LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int face, int bx, int by, int bz, int nx_,
+                      int ny_, int nz_, long long fc) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q, QS = L::QS;
+    const long long N = P.N;
+    const long long b = ((long long)bx * P.ny + by) * P.nz + bz;
+    const long long nb = ((long long)nx_ * P.ny + ny_) * P.nz + nz_;
+    const long long sb = band_index<D>(bx, by, bz, P.nx, P.ny, P.nz, 2);
+    const long long snb = band_index<D>(nx_, ny_, nz_, P.nx, P.ny, P.nz, 2);
+    Real ux = B.u[b], uy = B.u[N + b], uz = B.u[2 * N + b], rho = B.rho[b];
+    const Real nux = B.u[nb], nuy = B.u[N + nb], nuz = B.u[2 * N + nb], nrho = B.rho[nb];
+    const int bv = P.bc_v[face], br = P.bc_rho[face];
+    if (bv == BC_FIXED) {
+        const Real* vp = B.vprof + B.prof_off[face] * 3 + fc * 3;
+        ux = vp[0];
+        uy = vp[1];
+        uz = vp[2];
+    } else if (bv == BC_ZEROGRAD) {
+        ux = nux;
+        uy = nuy;
+        uz = nuz;
+    }
+    if (br == BC_FIXED) {
+        rho = (D == 2 && face == 0) ? B.rprof[B.prof_off[0] + fc] : P.rho_bc[face];
+    } else if (br == BC_ZEROGRAD) {
+        if (!(D == 3 && face == 4)) rho = nrho;
+    }
+    B.rho[b] = rho;
+    B.u[b] = ux;
+    B.u[N + b] = uy;
+    B.u[2 * N + b] = uz;
+    const Real eb = 1 - B.solid[b], enb = 1 - B.solid[nb];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Real fn = B.side[(long long)k * P.nshell + snb];
+        B.side[(long long)k * P.nshell + sb] =
+            feq<D, Real>(P.eos, k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(P.eos, k, nrho, nux, nuy, nuz, enb));
+    }
+    if (P.chem) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            if (j < P.nf) {
+                const SpecP<Real>& s = P.sp[j];
+                Real Sb = B.S[(long long)j * N + b];
+                const Real Sn = B.S[(long long)j * N + nb];
+                if (s.bc_type[face] == BC_FIXED) Sb = s.bc_value[face];
+                if (s.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
+                B.S[(long long)j * N + b] = Sb;
+                const int base = P.slot_sp + j * QS;
+#pragma unroll
+                for (int k = 0; k < QS; ++k) {
+                    const Real gn = B.side[(long long)(base + k) * P.nshell + snb];
+                    B.side[(long long)(base + k) * P.nshell + sb] =
+                        geq_sp<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_sp<D, Real>(k, Sn, nux, nuy, nuz));
+                }
+            }
+        }
+    }
+    if (P.thermal) {
+        {
+            Real Sb = B.TF[b];
+            const Real Sn = B.TF[nb];
+            if (P.tf.bc_type[face] == BC_FIXED) Sb = P.tf.bc_value[face];
+            if (P.tf.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
+            B.TF[b] = Sb;
+#pragma unroll
+            for (int k = 0; k < QS; ++k) {
+                const Real gn = B.side[(long long)(P.slot_tf + k) * P.nshell + snb];
+                B.side[(long long)(P.slot_tf + k) * P.nshell + sb] =
+                    geq_tf<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_tf<D, Real>(k, Sn, nux, nuy, nuz));
+            }
+        }
+        {
+            Real Sb = B.TS[b];
+            const Real Sn = B.TS[nb];
+            if (P.ts.bc_type[face] == BC_FIXED) Sb = P.ts.bc_value[face];
+            if (P.ts.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
+            B.TS[b] = Sb;
+#pragma unroll
+            for (int k = 0; k < QS; ++k) {
+                const Real gn = B.side[(long long)(P.slot_ts + k) * P.nshell + snb];
+                const Real w = (Real)L::ws(k);
+                B.side[(long long)(P.slot_ts + k) * P.nshell + sb] = w * Sb + (gn - w * Sn);
+            }
+        }
+    }
+}
+
+// collision n+1 of a face cell from its post-NEE state
+template <class Real, class Store, int D, int NF, int NX, int PAR>
+LBM_DEV void face_collide_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long fi) {
+    using L = Lat<D>;
+    constexpr int QS = L::QS;
+    const long long N = P.N;
+    Geo g;
+    g.nx = P.nx;
+    g.ny = P.ny;
+    g.nz = P.nz;
+    band_coord<D>(fi, P.nx, P.ny, P.nz, 1, g.x, g.y, g.z);
+    g.c = ((long long)g.x * P.ny + g.y) * P.nz + g.z;
+    g.edge = true;
+    const long long c = g.c;
+    const uint32_t mask = B.mask[c];
+    const bool pflow = (mask & 1u) != 0;
+    const bool pts = ((mask >> TS_BIT) & 1u) != 0;
+    const long long si = band_index<D>(g.x, g.y, g.z, P.nx, P.ny, P.nz, 2);
+    CellPops<Real, D, NF, NX> p;
+    CellMac<Real, NF, NX> m;
+    side_get_all<Real, Store, D, NF, NX>(P, B, si, p);
+    mac_get<Real, Store, NF, NX>(P, B, c, m);
+    load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        if (P.chem && j < P.nfix) m.S[NF + j] = B.fixS[(long long)j * N + c];
+    if (!pflow) {
+        store_fixed<L::Q>(B.pops, N, 0, c, p.f);
+        if (P.thermal) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (P.chem && j < P.nf) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+        if (P.thermal) {
+            collide_emitter<Real, D, NF, NX>(P, p.gS, m);
+#pragma unroll
+            for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = p.gS[k];
+            scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+        }
+        return;
+    }
+    lag_put<Real, Store, NF, NX>(P, B, c, m);
+    collide<Real, D, NF, NX>(P, p, m, pts);
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
+    scatter_all<D, PAR, Real, Store, NF, NX>(P, B, g, mask, p, pts);
+}
+
+// Equilibrium initialisation (LBM3D/_initialize.py:45-77) followed by the
+// collision of step 1, written with the even-step layout.
+template <class Real, class Store, int D, int NF, int NX>
+LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, int inflow_x2, Real v0) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q, QS = L::QS;
+    const long long N = P.N;
+    const uint32_t mask = B.mask[c];
+    Geo g;
+    g.c = c;
+    g.nx = P.nx;
+    g.ny = P.ny;
+    g.nz = P.nz;
+    cell_coord(c, P.ny, P.nz, g.x, g.y, g.z);
+    g.edge = (mask & FACE_BIT) != 0;
+    const bool pflow = (mask & 1u) != 0;
+    const bool pts = ((mask >> TS_BIT) & 1u) != 0;
+    CellPops<Real, D, NF, NX> p;
+    CellMac<Real, NF, NX> m;
+    mac_get<Real, Store, NF, NX>(P, B, c, m);
+    load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+    if (P.chem && P.poro) {
+        Real rhos = (Real)0.0;
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (j < P.nfix) rhos += m.S[NF + j];
+        m.rhos = rhos;
+        B.rhos[c] = rhos;
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) p.f[k] = feq<D, Real>(P.eos, k, m.rho, m.ux, m.uy, m.uz, m.eps);
+#pragma unroll
+    for (int k = 0; k < QS; ++k) {
+        p.gT[k] = geq_tf<D, Real>(k, m.TF, m.ux, m.uy, m.uz);
+        p.gS[k] = (Real)L::ws(k) * m.TS;
+#pragma unroll
+        for (int j = 0; j < NF; ++j) p.gP[j][k] = geq_sp<D, Real>(k, m.S[j], m.ux, m.uy, m.uz);
+    }
+    // init_boundary of an inflow-rate inlet sets v of the first two columns after the populations
+    if (inflow_x2 && g.x < 2) {
+        m.ux = v0;
+        m.uy = (Real)0;
+        m.uz = (Real)0;
+        B.u[c] = m.ux;
+        B.u[N + c] = m.uy;
+        B.u[2 * N + c] = m.uz;
+    }
+    if (!pflow) {
+        store_fixed<Q>(B.pops, N, 0, c, p.f);
+        if (P.thermal) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (P.chem && j < P.nf) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
+        if (P.thermal) {
+            collide_emitter<Real, D, NF, NX>(P, p.gS, m);
+#pragma unroll
+            for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = p.gS[k];
+            scatter<D, QS, 0>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+        }
+        return;
+    }
+    lag_put<Real, Store, NF, NX>(P, B, c, m);
+    collide<Real, D, NF, NX>(P, p, m, pts);
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+        if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
+    scatter_all<D, 0, Real, Store, NF, NX>(P, B, g, mask, p, pts);
+}
+
+// per-cell flags (solid is static after init)
+template <class Real, int D>
+LBM_DEV uint32_t make_mask(const Real* __restrict__ solid, long long c, int nx, int ny, int nz) {
+    using L = Lat<D>;
+    Geo g;
+    g.c = c;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    cell_coord(c, ny, nz, g.x, g.y, g.z);
+    const bool face = g.x == 0 || g.x == nx - 1 || g.y == 0 || g.y == ny - 1 || (D == 3 && (g.z == 0 || g.z == nz - 1));
+    const bool shell = g.x <= 1 || g.x >= nx - 2 || g.y <= 1 || g.y >= ny - 2 || (D == 3 && (g.z <= 1 || g.z >= nz - 2));
+    g.edge = face;
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < L::Q; ++k)
+        if (solid[nb_of<D>(g, k)] < 1) m |= 1u << k;
+#pragma unroll
+    for (int k = 0; k < L::QS; ++k)
+        if (solid[nb_of<D>(g, k)] > 0) m |= 1u << (TS_BIT + k);
+    if (face) m |= FACE_BIT;
+    if (shell) m |= SHELL_BIT;
+    if (solid[c] != 0) m |= SOLIDNZ_BIT;
+    return m;
+}
+
+}  // namespace lbm

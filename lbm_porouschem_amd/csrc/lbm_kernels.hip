@@ -1,0 +1,864 @@
+// lbm_kernels.hip -- gfx950 kernels and the C ABI (include/lbm_porouschem.h) of the
+// coupled lattice-Boltzmann step.  The per-cell logic lives in lbm_core.hpp; this
+// file holds the __global__ wrappers, device memory management and the host
+// side of every entry point.
+//
+// Kernel sequence of one time step (all on the handle's stream):
+//   [k_inflow2d]   2D inflow-rate controller (LBM2D/_boundary.py:324-333)
+//   k_bulk<par>    streaming (A-A gather) + macro + collision/scatter of non-face cells
+//   k_nee x,y[,z]  non-equilibrium extrapolation of the face cells, face order of
+//                  LBM3D/_boundary.py:155-208
+//   k_face<par>    collision/scatter of the face cells from the post-BC state
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "../../include/lbm_porouschem.h"
+#include "lbm_core.hpp"
+
+using namespace lbm;
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <class Real, int D>
+__global__ __launch_bounds__(BLOCK) void k_mask(const Real* __restrict__ solid, uint32_t* __restrict__ mask, long long N,
+                                                int nx, int ny, int nz) {
+    long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= N) return;
+    mask[c] = make_mask<Real, D>(solid, c, nx, ny, nz);
+}
+
+template <class Real, class Store, int D, int NF, int NX>
+__global__ __launch_bounds__(BLOCK) void k_init(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int inflow,
+                                                Real v0) {
+    const DevParams<Real>& P = *Pp;
+    long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= P.N) return;
+    init_cell<Real, Store, D, NF, NX>(P, B, c, inflow, v0);
+}
+
+template <class Real, class Store, int D, int NF, int NX, int PAR>
+__global__ __launch_bounds__(BLOCK) void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B,
+                                                int write_macro) {
+    const DevParams<Real>& P = *Pp;
+    long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= P.N) return;
+    bulk_cell<Real, Store, D, NF, NX, PAR>(P, B, c, write_macro);
+}
+
+// family 0: x faces (threads over y,z), 1: y faces (x,z), 2: z faces (x,y)
+template <class Real, class Store, int D, int NF, int NX>
+__global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int family) {
+    const DevParams<Real>& P = *Pp;
+    long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    if (family == 0) {
+        if (t >= (long long)ny * nz) return;
+        int y = (int)(t / nz), z = (int)(t % nz);
+        long long fc = (long long)y * nz + z;
+        nee_cell<Real, Store, D, NF, NX>(P, B, 0, 0, y, z, 1, y, z, fc);
+        nee_cell<Real, Store, D, NF, NX>(P, B, 1, nx - 1, y, z, nx - 2, y, z, fc);
+    } else if (family == 1) {
+        if (t >= (long long)nx * nz) return;
+        int x = (int)(t / nz), z = (int)(t % nz);
+        long long fc = (long long)x * nz + z;
+        nee_cell<Real, Store, D, NF, NX>(P, B, 2, x, 0, z, x, 1, z, fc);
+        nee_cell<Real, Store, D, NF, NX>(P, B, 3, x, ny - 1, z, x, ny - 2, z, fc);
+    } else {
+        if (t >= (long long)nx * ny) return;
+        int x = (int)(t / ny), y = (int)(t % ny);
+        long long fc = (long long)x * ny + y;
+        nee_cell<Real, Store, D, NF, NX>(P, B, 4, x, y, 0, x, y, 1, fc);
+        nee_cell<Real, Store, D, NF, NX>(P, B, 5, x, y, nz - 1, x, y, nz - 2, fc);
+    }
+}
+
+template <class Real, class Store, int D, int NF, int NX, int PAR>
+__global__ __launch_bounds__(BLOCK) void k_face(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B) {
+    const DevParams<Real>& P = *Pp;
+    long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= P.nface) return;
+    face_collide_cell<Real, Store, D, NF, NX, PAR>(P, B, t);
+}
+
+// 2D inflow-rate controller: one block sums rho*u_x over the inlet column in a
+// fixed order, then rescales the profile from column x = 1.
+template <class Real, class Store>
+__global__ __launch_bounds__(BLOCK) void k_inflow2d(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B,
+                                                    Real flow_bc) {
+    const DevParams<Real>& P = *Pp;
+    __shared__ Real part[BLOCK];
+    const int ny = P.ny;
+    Real acc = (Real)0.0;
+    for (int y = threadIdx.x; y < ny; y += BLOCK) acc += B.rho[y] * B.u[y];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Real s = (Real)0.0;
+        for (int i = 0; i < BLOCK; ++i) s += part[i];
+        part[0] = s;
+    }
+    __syncthreads();
+    const Real coef = flow_bc / part[0];
+    const Real a = (Real).01;
+    const long long N = P.N;
+    for (int y = threadIdx.x; y < ny; y += BLOCK) {
+        long long n1 = (long long)1 * ny + y;
+        Real* vp = B.vprof + B.prof_off[0] * 3 + (long long)y * 3;
+        vp[0] = (a * coef + 1 - a) * B.u[n1];
+        vp[1] = (a * coef + 1 - a) * B.u[N + n1];
+        vp[2] = (a * coef + 1 - a) * B.u[2 * N + n1];
+        B.rprof[B.prof_off[0] + y] = B.rho[n1];
+    }
+}
+
+// FP64 reductions over the macroscopic fields with wavefront shuffles.
+// kind: LBM_REDUCE_*; partials[blockIdx] receives the block result.
+template <class Real, class Store>
+__global__ __launch_bounds__(BLOCK) void k_reduce(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int kind,
+                                                  int comp, double* __restrict__ partials) {
+    const DevParams<Real>& P = *Pp;
+    const long long N = P.N;
+    const bool isMax = kind == LBM_REDUCE_MAX_V || kind == LBM_REDUCE_MAX_TF;
+    const bool isMin = kind == LBM_REDUCE_MIN_TF;
+    double acc = isMax ? -1e10 : (isMin ? 1e10 : 0.0);
+    for (long long c = (long long)blockIdx.x * BLOCK + threadIdx.x; c < N; c += (long long)gridDim.x * BLOCK) {
+        const Real sol = B.solid[c];
+        double v;
+        switch (kind) {
+            case LBM_REDUCE_MAX_V: {
+                double ux = B.u[c], uy = B.u[N + c], uz = B.u[2 * N + c];
+                v = sqrt(ux * ux + uy * uy + uz * uz);
+                if (!(v == v)) v = INFINITY;  // a NaN velocity counts as divergence
+                acc = fmax(acc, v);
+                break;
+            }
+            case LBM_REDUCE_MIN_TF: acc = fmin(acc, (double)B.TF[c]); break;
+            case LBM_REDUCE_MAX_TF: acc = fmax(acc, (double)B.TF[c]); break;
+            case LBM_REDUCE_MASS: if (sol < 1) acc += (double)B.rho[c]; break;
+            case LBM_REDUCE_TF_SUM: if (sol < 1) acc += (double)B.TF[c]; break;
+            case LBM_REDUCE_TS_SUM: if (sol > 0) acc += (double)B.TS[c]; break;
+            case LBM_REDUCE_FIX_SUM:
+                for (int j = 0; j < P.nfix; ++j) acc += (double)B.S[(long long)(P.nf + j) * N + c];
+                break;
+            case LBM_REDUCE_SPECIES_SUM: if (sol < 1) acc += (double)B.S[(long long)comp * N + c]; break;
+            default: break;
+        }
+    }
+    // wavefront (64 lanes) butterfly, then across the block's 4 wavefronts
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(acc, off, 64);
+        acc = isMax ? fmax(acc, o) : (isMin ? fmin(acc, o) : acc + o);
+    }
+    __shared__ double wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wsum[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = wsum[0];
+        for (int i = 1; i < BLOCK / 64; ++i) r = isMax ? fmax(r, wsum[i]) : (isMin ? fmin(r, wsum[i]) : r + wsum[i]);
+        partials[blockIdx.x] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Impl {
+    virtual ~Impl() {}
+    virtual int alloc() = 0;
+    virtual int set_params(const lbm_params& p) = 0;
+    virtual int set_field(int field, int comp, const void* host, int dtype, size_t n) = 0;
+    virtual int get_field(int field, int comp, void* host, int dtype, size_t n) = 0;
+    virtual int init() = 0;
+    virtual int step(int n) = 0;
+    virtual int reduce(int kind, int comp, double* out) = 0;
+    virtual int sync() = 0;
+    virtual int profile(int enable) = 0;
+    virtual int profile_read(int kernel, double* ms, long long* n) = 0;
+};
+
+}  // namespace
+
+struct lbm_handle {
+    lbm_desc d;
+    lbm_params p;
+    int params_set = 0, initialised = 0;
+    long long steps = 0;
+    char err[512];
+    std::unique_ptr<Impl> impl;
+};
+
+namespace {
+
+int set_err(lbm_handle* h, int code, const char* fmt, ...) {
+    if (h) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(h->err, sizeof(h->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return set_err(h_, LBM_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                           __FILE__, __LINE__);                                                          \
+    } while (0)
+
+inline unsigned grid_for(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+template <class Real, class Store, int D, int NF, int NX>
+struct ImplT final : Impl {
+    using L = Lat<D>;
+    static constexpr int Q = L::Q, QS = L::QS;
+    lbm_handle* h_;
+    hipStream_t stream = nullptr;
+    int nx, ny, nz;
+    long long N, nshell, nface;
+    int nf = 0, nfix = 0;
+    int dev_of[LBM_MAX_SPECIES];  // registration order -> device order
+    DevParams<Real> P{};
+    DevParams<Real>* dP = nullptr;
+    Bufs<Real, Store> B{};
+    std::vector<void*> owned;
+    // separate device copies of the reference's static fields
+    Real *exch_coef = nullptr, *exch_surf = nullptr, *rad_surf = nullptr, *emis = nullptr, *ha = nullptr, *rad = nullptr;
+    Real* solid = nullptr;
+    double* partials = nullptr;
+    bool has_emit = false;
+    // device-event timing per kernel class
+    bool prof_on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[4];
+    size_t prof_used[4] = {0, 0, 0, 0};
+    hipEvent_t ev_begin(int k) {
+        if (!prof_on) return nullptr;
+        auto& v = prof_ev[k];
+        if (prof_used[k] == v.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+            v.push_back({a, b});
+        }
+        hipEvent_t a = v[prof_used[k]].first;
+        (void)hipEventRecord(a, stream);
+        return a;
+    }
+    void ev_end(int k, hipEvent_t a) {
+        if (!a) return;
+        (void)hipEventRecord(prof_ev[k][prof_used[k]].second, stream);
+        prof_used[k]++;
+    }
+    long long prof_n[6] = {0, 0, 0, 0, 0, 0};
+
+    explicit ImplT(lbm_handle* h) : h_(h) {
+        nx = h->d.nx;
+        ny = h->d.ny;
+        nz = h->d.nz;
+        N = (long long)nx * ny * nz;
+        nshell = band_count<D>(nx, ny, nz, 2);
+        nface = band_count<D>(nx, ny, nz, 1);
+        for (int j = 0; j < h->d.n_species; ++j)
+            if (!h->d.species_fix[j]) dev_of[j] = nf++;
+        for (int j = 0; j < h->d.n_species; ++j)
+            if (h->d.species_fix[j]) dev_of[j] = nf + nfix++;
+    }
+    ~ImplT() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (int k = 0; k < 4; ++k)
+            for (auto& e : prof_ev[k]) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+        for (void* p : owned) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T> int dalloc(T** p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return set_err(h_, LBM_ERR_OOM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        owned.push_back(*p);
+        e = hipMemsetAsync(*p, 0, count * sizeof(T), stream);
+        if (e != hipSuccess) return set_err(h_, LBM_ERR_DEVICE, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+        return LBM_OK;
+    }
+
+    int alloc() override {
+        HIP_TRY(hipSetDevice(h_->d.device));
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        const int therm = h_->d.thermal, chem = h_->d.chemical;
+        P.slot_tf = Q;
+        P.slot_ts = Q + (therm ? QS : 0);
+        P.slot_sp = Q + (therm ? 2 * QS : 0);
+        P.nslots = P.slot_sp + (chem ? nf * QS : 0);
+        int rc;
+        if ((rc = dalloc(&B.pops, (size_t)P.nslots * N))) return rc;
+        if ((rc = dalloc(&B.side, (size_t)P.nslots * nshell))) return rc;
+        uint32_t* mask;
+        if ((rc = dalloc(&mask, (size_t)N))) return rc;
+        B.mask = mask;
+        if ((rc = dalloc(&solid, (size_t)N))) return rc;
+        B.solid = solid;
+        Real *cd, *cf;
+        if ((rc = dalloc(&cd, h_->d.poro ? (size_t)N : 1))) return rc;
+        if ((rc = dalloc(&cf, h_->d.poro ? (size_t)N : 1))) return rc;
+        B.cd = cd;
+        B.cf = cf;
+        const size_t NT = therm ? (size_t)N : 1;
+        if ((rc = dalloc(&exch_coef, NT)) || (rc = dalloc(&exch_surf, NT)) || (rc = dalloc(&rad_surf, NT)) ||
+            (rc = dalloc(&emis, NT)) || (rc = dalloc(&ha, NT)) || (rc = dalloc(&rad, NT)))
+            return rc;
+        B.ha = ha;
+        B.rad = rad;
+        if ((rc = dalloc(&B.fixS, (size_t)(nfix > 0 ? nfix : 1) * (nfix > 0 ? N : 1)))) return rc;
+        if ((rc = dalloc(&B.lag, (therm && chem) ? (size_t)N : 1))) return rc;
+        if ((rc = dalloc(&B.rho, (size_t)N)) || (rc = dalloc(&B.u, 3 * (size_t)N)) || (rc = dalloc(&B.TF, NT)) ||
+            (rc = dalloc(&B.TS, NT)) || (rc = dalloc(&B.dTF, NT)) || (rc = dalloc(&B.dTS, NT)) ||
+            (rc = dalloc(&B.rhos, NT)))
+            return rc;
+        const int ns = h_->d.n_species;
+        if ((rc = dalloc(&B.S, ns > 0 ? (size_t)ns * N : 1)) || (rc = dalloc(&B.dS, ns > 0 ? (size_t)ns * N : 1)))
+            return rc;
+        B.rhos_static = B.rhos;
+        long long off = 0;
+        for (int f = 0; f < 2 * D; ++f) {
+            prof_n[f] = f < 2 ? (long long)ny * nz : (f < 4 ? (long long)nx * nz : (long long)nx * ny);
+            B.prof_off[f] = off;
+            off += prof_n[f];
+        }
+        if ((rc = dalloc(&B.vprof, 3 * (size_t)off)) || (rc = dalloc(&B.rprof, (size_t)off))) return rc;
+        if ((rc = dalloc(&dP, 1))) return rc;
+        if ((rc = dalloc(&partials, 4096))) return rc;
+        // rho = 1 (default_init, LBM3D/_initialize.py:8-16)
+        std::vector<Real> ones((size_t)N, (Real)1);
+        HIP_TRY(hipMemcpyAsync(B.rho, ones.data(), (size_t)N * sizeof(Real), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    int set_params(const lbm_params& p) override {
+        const lbm_desc& d = h_->d;
+        const double dtdx2 = p.dt / (p.dx * p.dx);
+        auto LU = [&](double v) { return (Real)(v * p.dt / (p.dx * p.dx)); };
+        DevParams<Real> Q_{};
+        Q_.dim = D;
+        Q_.nx = nx;
+        Q_.ny = ny;
+        Q_.nz = nz;
+        Q_.N = N;
+        Q_.nshell = nshell;
+        Q_.nface = nface;
+        Q_.thermal = d.thermal;
+        Q_.chem = d.chemical;
+        Q_.poro = d.poro;
+        Q_.radiation = d.radiation && d.thermal;
+        Q_.nf = nf;
+        Q_.nfix = nfix;
+        Q_.eos = p.eos;
+        Q_.visc_model = p.visc_model;
+        Q_.force_guo = p.force_term == LBM_FORCE_GUO;
+        Q_.poro_model = p.poro_model;
+        Q_.rad_model = p.radiation_model;
+        Q_.bc_nee = p.bc_model == LBM_BCMODEL_NEE;
+        for (int f = 0; f < 6; ++f) {
+            Q_.bc_v[f] = p.bc_v[f];
+            Q_.bc_rho[f] = p.bc_rho[f];
+            Q_.rho_bc[f] = (Real)p.rho_bc[f];
+        }
+        Q_.slot_tf = P.slot_tf;
+        Q_.slot_ts = P.slot_ts;
+        Q_.slot_sp = P.slot_sp;
+        Q_.nslots = P.nslots;
+        Q_.tol = (Real)p.tol;
+        Q_.dt = (Real)p.dt;
+        Q_.dx = (Real)p.dx;
+        Q_.dtdx2 = (Real)dtdx2;
+        Q_.visc_lu = LU(p.visco);
+        Q_.suth0 = (Real)p.sutherland[0];
+        Q_.suth1 = (Real)p.sutherland[1];
+        Q_.t_amb4 = (Real)pow(p.t_ambient, 4);
+        auto therm = [&](ThermP<Real>& t, const lbm_thermal_params& s) {
+            t.diff_model = s.diff_model;
+            t.cond_model = s.cond_model;
+            t.cap_model = s.cap_model;
+            t.v_ref = (Real)s.v_ref;
+            t.v_scale = (Real)s.v_scale;
+            t.diff_lu = LU(s.diff);
+            t.Pr = (Real)s.Pr;
+            t.cond = (Real)s.cond;
+            t.cm = (Real)s.cm;
+            t.T1 = (Real)s.Trange[1];
+            for (int i = 0; i < 5; ++i) {
+                t.cond_poly[i] = (Real)s.cond_poly[i];
+                t.cm_poly[i] = (Real)s.cm_poly[i];
+                t.nasa[0][i] = (Real)s.nasa[0][i];
+                t.nasa[1][i] = (Real)s.nasa[1][i];
+            }
+            for (int f = 0; f < 6; ++f) {
+                t.bc_type[f] = s.bc_type[f];
+                t.bc_value[f] = (Real)s.bc_value[f];
+            }
+        };
+        therm(Q_.tf, p.tf);
+        therm(Q_.ts, p.ts);
+        bool any_sp_suth = false;
+        for (int j = 0; j < d.n_species; ++j) {
+            const lbm_species_params& s = p.species[j];
+            SpecP<Real>& t = Q_.sp[dev_of[j]];
+            t.visc_model = s.visc_model;
+            t.diff_model = s.diff_model;
+            t.cond_model = s.cond_model;
+            t.molemass = (Real)s.molemass;
+            t.visc_lu = LU(s.visco);
+            t.suth0 = (Real)s.sutherland[0];
+            t.suth1 = (Real)s.sutherland[1];
+            t.diff_lu = LU(s.diff);
+            t.Sc = (Real)s.Sc;
+            t.enthalpy = (Real)s.enthalpy;
+            t.cap_m = (Real)(s.capa / s.molemass);
+            t.cond = (Real)s.cond;
+            for (int i = 0; i < 5; ++i) t.cond_poly[i] = (Real)s.cond_poly[i];
+            for (int f = 0; f < 6; ++f) {
+                t.bc_type[f] = s.bc_type[f];
+                t.bc_value[f] = (Real)s.bc_value[f];
+            }
+            if (!d.species_fix[j] && s.visc_model == LBM_VISC_SUTHERLAND) any_sp_suth = true;
+        }
+        Q_.n_reac = p.n_reactions;
+        for (int r = 0; r < p.n_reactions; ++r) {
+            const lbm_reaction_params& s = p.reactions[r];
+            ReacP<Real>& t = Q_.r[r];
+            t.A = (Real)s.A;
+            t.Ea = (Real)s.Ea;
+            t.b = (Real)s.b;
+            t.Tmin = (Real)s.Tmin;
+            for (int j = 0; j < d.n_species; ++j) {
+                t.reac[dev_of[j]] = (Real)s.coef_reactant[j];
+                t.prod[dev_of[j]] = (Real)s.coef_product[j];
+                t.rate[dev_of[j]] = (Real)s.coef_rate[j];
+            }
+        }
+        // old-state reads of the porous velocity correction (LBM3D/_evolution.py:130-150)
+        Q_.lag_tf = d.thermal && d.poro &&
+                    (p.visc_model == LBM_VISC_SUTHERLAND || (p.visc_model == LBM_VISC_MIXTURE && d.chemical && any_sp_suth));
+        Q_.lag_sp = d.poro && d.chemical && p.visc_model == LBM_VISC_MIXTURE && nf > 0;
+        if (Q_.lag_tf && !B.lagTF) {
+            int rc = dalloc(&B.lagTF, (size_t)N);
+            if (rc) return rc;
+        }
+        if (Q_.lag_sp && !B.lagS) {
+            int rc = dalloc(&B.lagS, (size_t)nf * N);
+            if (rc) return rc;
+        }
+        if (p.n_reactions > MAXR) return set_err(h_, LBM_ERR_UNSUPPORTED, "too many reactions");
+        P = Q_;
+        HIP_TRY(hipMemcpyAsync(dP, &P, sizeof(P), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    // field id -> device pointer / element count; face profiles hold 3 values per face cell
+    int locate(int field, int comp, Real** ptr, long long* count, long long* stride) {
+        *stride = 1;
+        *count = N;
+        const int ns = h_->d.n_species;
+        const bool T = h_->d.thermal;
+        switch (field) {
+            case LBM_FIELD_RHO: *ptr = B.rho; return LBM_OK;
+            case LBM_FIELD_VEL:
+                if (comp < 0 || comp > 2) break;
+                *ptr = B.u + (long long)comp * N;
+                return LBM_OK;
+            case LBM_FIELD_SOLID: *ptr = solid; return LBM_OK;
+            case LBM_FIELD_RHOS: if (!T) break; *ptr = B.rhos; return LBM_OK;
+            case LBM_FIELD_COEF_DARCY: if (!h_->d.poro) break; *ptr = (Real*)B.cd; return LBM_OK;
+            case LBM_FIELD_COEF_FORCH: if (!h_->d.poro) break; *ptr = (Real*)B.cf; return LBM_OK;
+            case LBM_FIELD_TF: if (!T) break; *ptr = B.TF; return LBM_OK;
+            case LBM_FIELD_TS: if (!T) break; *ptr = B.TS; return LBM_OK;
+            case LBM_FIELD_TF_DS: if (!T) break; *ptr = B.dTF; return LBM_OK;
+            case LBM_FIELD_TS_DS: if (!T) break; *ptr = B.dTS; return LBM_OK;
+            case LBM_FIELD_EXCH_SURF: if (!T) break; *ptr = exch_surf; return LBM_OK;
+            case LBM_FIELD_EXCH_COEF: if (!T) break; *ptr = exch_coef; return LBM_OK;
+            case LBM_FIELD_RAD_SURF: if (!T) break; *ptr = rad_surf; return LBM_OK;
+            case LBM_FIELD_EMISSIVITY: if (!T) break; *ptr = emis; return LBM_OK;
+            case LBM_FIELD_SPECIES:
+                if (comp < 0 || comp >= ns) break;
+                *ptr = B.S + (long long)dev_of[comp] * N;
+                return LBM_OK;
+            case LBM_FIELD_SPECIES_DS:
+                if (comp < 0 || comp >= ns) break;
+                *ptr = B.dS + (long long)dev_of[comp] * N;
+                return LBM_OK;
+            case LBM_FIELD_V_BC_PROFILE: {
+                int f = comp / 3, ax = comp % 3;
+                if (comp < 0 || f >= 2 * D) break;
+                *ptr = B.vprof + B.prof_off[f] * 3 + ax;
+                *count = prof_n[f];
+                *stride = 3;
+                return LBM_OK;
+            }
+            case LBM_FIELD_RHO_BC_PROFILE:
+                if (comp < 0 || comp >= 2 * D) break;
+                *ptr = B.rprof + B.prof_off[comp];
+                *count = prof_n[comp];
+                return LBM_OK;
+            default: break;
+        }
+        return set_err(h_, LBM_ERR_INVALID, "field %d comp %d is not available for this solver", field, comp);
+    }
+
+    int set_field(int field, int comp, const void* host, int dtype, size_t n) override {
+        Real* ptr;
+        long long count, stride;
+        int rc = locate(field, comp, &ptr, &count, &stride);
+        if (rc) return rc;
+        if ((long long)n != count)
+            return set_err(h_, LBM_ERR_INVALID, "field %d expects %lld values, got %zu", field, count, n);
+        if (h_->initialised && field != LBM_FIELD_V_BC_PROFILE && field != LBM_FIELD_RHO_BC_PROFILE)
+            return set_err(h_, LBM_ERR_STATE,
+                           "field %d feeds the populations; set it before lbm_init (or call lbm_init again)", field);
+        std::vector<Real> tmp(n);
+        for (size_t i = 0; i < n; ++i)
+            tmp[i] = dtype == LBM_DTYPE_F64 ? (Real)((const double*)host)[i] : (Real)((const float*)host)[i];
+        if (stride == 1) {
+            HIP_TRY(hipMemcpyAsync(ptr, tmp.data(), n * sizeof(Real), hipMemcpyHostToDevice, stream));
+        } else {
+            HIP_TRY(hipMemcpy2DAsync(ptr, stride * sizeof(Real), tmp.data(), sizeof(Real), sizeof(Real), n,
+                                     hipMemcpyHostToDevice, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    int get_field(int field, int comp, void* host, int dtype, size_t n) override {
+        Real* ptr;
+        long long count, stride;
+        int rc = locate(field, comp, &ptr, &count, &stride);
+        if (rc) return rc;
+        if ((long long)n != count)
+            return set_err(h_, LBM_ERR_INVALID, "field %d holds %lld values, asked %zu", field, count, n);
+        std::vector<Real> tmp(n);
+        if (stride == 1) {
+            HIP_TRY(hipMemcpyAsync(tmp.data(), ptr, n * sizeof(Real), hipMemcpyDeviceToHost, stream));
+        } else {
+            HIP_TRY(hipMemcpy2DAsync(tmp.data(), sizeof(Real), ptr, stride * sizeof(Real), sizeof(Real), n,
+                                     hipMemcpyDeviceToHost, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < n; ++i) {
+            if (dtype == LBM_DTYPE_F64) ((double*)host)[i] = (double)tmp[i];
+            else ((float*)host)[i] = (float)tmp[i];
+        }
+        return LBM_OK;
+    }
+
+    int init() override {
+        const lbm_params& p = h_->p;
+        const lbm_desc& d = h_->d;
+        // static products used by the step: exchangeCoef*exchangeSurface, and the radiation prefactor
+        if (d.thermal) {
+            std::vector<Real> a((size_t)N), b((size_t)N), c((size_t)N), e((size_t)N);
+            HIP_TRY(hipMemcpyAsync(a.data(), exch_coef, N * sizeof(Real), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(b.data(), exch_surf, N * sizeof(Real), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(c.data(), emis, N * sizeof(Real), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(e.data(), rad_surf, N * sizeof(Real), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            for (long long i = 0; i < N; ++i) {
+                a[i] = a[i] * b[i];
+                c[i] = c[i] * (Real)SIGMA_SB * e[i] / (Real)p.dx;
+            }
+            HIP_TRY(hipMemcpyAsync(ha, a.data(), N * sizeof(Real), hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(rad, c.data(), N * sizeof(Real), hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
+        // fully solid cells need persistent TS populations
+        {
+            std::vector<Real> s((size_t)N);
+            HIP_TRY(hipMemcpyAsync(s.data(), solid, N * sizeof(Real), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            bool any = false;
+            for (long long i = 0; i < N && !any; ++i) any = !(s[i] < 1);
+            has_emit = any && d.thermal;
+            if (has_emit && !B.emit) {
+                int rc = dalloc(&B.emit, (size_t)QS * N);
+                if (rc) return rc;
+            }
+        }
+        hipLaunchKernelGGL((k_mask<Real, D>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, solid, (uint32_t*)B.mask, N,
+                           nx, ny, nz);
+        HIP_TRY(hipGetLastError());
+        // inlet profile of face 0 (init_boundary, LBM3D/_initialize.py:79-88)
+        int inflow = p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW;
+        Real v0 = (Real)(p.flow_bc[0] / ny / nz);
+        if (inflow || p.bc_flow[0] == LBM_BCFLOW_INLET) {
+            std::vector<Real> vp((size_t)3 * prof_n[0]);
+            for (long long i = 0; i < prof_n[0]; ++i) {
+                vp[3 * i] = inflow ? v0 : (Real)p.v_bc[0][0];
+                vp[3 * i + 1] = inflow ? (Real)0 : (Real)p.v_bc[0][1];
+                vp[3 * i + 2] = inflow ? (Real)0 : (Real)p.v_bc[0][2];
+            }
+            HIP_TRY(hipMemcpyAsync(B.vprof, vp.data(), vp.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
+        }
+        hipLaunchKernelGGL((k_init<Real, Store, D, NF, NX>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, dP, B, inflow,
+                           v0);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    int step(int n) override {
+        const lbm_params& p = h_->p;
+        for (int s = 0; s < n; ++s) {
+            const long long stepno = h_->steps + 1;  // step 1 reads the layout written by the init collision
+            const int wm = s == n - 1;
+            hipEvent_t e_step = ev_begin(3);
+            if (D == 2 && p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW)
+                hipLaunchKernelGGL((k_inflow2d<Real, Store>), dim3(1), dim3(BLOCK), 0, stream, dP, B, (Real)p.flow_bc[0]);
+            hipEvent_t e_bulk = ev_begin(0);
+            if (stepno & 1)
+                hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, dP, B, wm);
+            else
+                hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, dP, B, wm);
+            ev_end(0, e_bulk);
+            hipEvent_t e_nee = ev_begin(1);
+            if (p.bc_model == LBM_BCMODEL_NEE) {
+                hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for((long long)ny * nz)), dim3(BLOCK), 0,
+                                   stream, dP, B, 0);
+                hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for((long long)nx * nz)), dim3(BLOCK), 0,
+                                   stream, dP, B, 1);
+                if (D == 3)
+                    hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for((long long)nx * ny)), dim3(BLOCK),
+                                       0, stream, dP, B, 2);
+            }
+            ev_end(1, e_nee);
+            hipEvent_t e_face = ev_begin(2);
+            if (stepno & 1)
+                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1>), dim3(grid_for(nface)), dim3(BLOCK), 0, stream, dP, B);
+            else
+                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0>), dim3(grid_for(nface)), dim3(BLOCK), 0, stream, dP, B);
+            ev_end(2, e_face);
+            ev_end(3, e_step);
+            HIP_TRY(hipGetLastError());
+            h_->steps++;
+        }
+        return LBM_OK;
+    }
+
+    int reduce(int kind, int comp, double* out) override {
+        if (kind < 0 || kind > LBM_REDUCE_SPECIES_SUM) return set_err(h_, LBM_ERR_INVALID, "unknown reduction %d", kind);
+        const lbm_desc& d = h_->d;
+        if ((kind == LBM_REDUCE_MIN_TF || kind == LBM_REDUCE_MAX_TF || kind == LBM_REDUCE_TF_SUM ||
+             kind == LBM_REDUCE_TS_SUM) && !d.thermal)
+            return set_err(h_, LBM_ERR_INVALID, "thermal module is off");
+        int dcomp = 0;
+        if (kind == LBM_REDUCE_SPECIES_SUM) {
+            if (comp < 0 || comp >= d.n_species) return set_err(h_, LBM_ERR_INVALID, "species %d", comp);
+            dcomp = dev_of[comp];
+        }
+        unsigned grid = (unsigned)std::min<long long>(grid_for(N), 1024);
+        hipLaunchKernelGGL((k_reduce<Real, Store>), dim3(grid), dim3(BLOCK), 0, stream, dP, B, kind, dcomp, partials);
+        HIP_TRY(hipGetLastError());
+        std::vector<double> part(grid);
+        HIP_TRY(hipMemcpyAsync(part.data(), partials, grid * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        double r = part[0];
+        for (unsigned i = 1; i < grid; ++i) {
+            if (kind == LBM_REDUCE_MAX_V || kind == LBM_REDUCE_MAX_TF) r = std::max(r, part[i]);
+            else if (kind == LBM_REDUCE_MIN_TF) r = std::min(r, part[i]);
+            else r += part[i];
+        }
+        *out = r;
+        return LBM_OK;
+    }
+
+    int sync() override {
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    int profile(int enable) override {
+        HIP_TRY(hipStreamSynchronize(stream));
+        prof_on = enable != 0;
+        if (prof_on)
+            for (int k = 0; k < 4; ++k) prof_used[k] = 0;
+        return LBM_OK;
+    }
+
+    int profile_read(int kernel, double* ms, long long* n) override {
+        if (kernel < 0 || kernel > 3) return set_err(h_, LBM_ERR_INVALID, "kernel class %d", kernel);
+        HIP_TRY(hipStreamSynchronize(stream));
+        double tot = 0;
+        for (size_t i = 0; i < prof_used[kernel]; ++i) {
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, prof_ev[kernel][i].first, prof_ev[kernel][i].second));
+            tot += t;
+        }
+        *ms = tot;
+        *n = (long long)prof_used[kernel];
+        return LBM_OK;
+    }
+};
+
+template <class Real, class Store, int D>
+Impl* make_for_species(lbm_handle* h, int nf, int nfix) {
+    if (nf == 0 && nfix == 0) return new ImplT<Real, Store, D, 0, 0>(h);
+    if (nf <= 4 && nfix <= 2) return new ImplT<Real, Store, D, 4, 2>(h);
+    if (nf <= 8 && nfix <= 4) return new ImplT<Real, Store, D, 8, 4>(h);
+    return nullptr;
+}
+
+Impl* make_impl(lbm_handle* h) {
+    const lbm_desc& d = h->d;
+    int nf = 0, nfix = 0;
+    if (d.chemical)
+        for (int j = 0; j < d.n_species; ++j) (d.species_fix[j] ? nfix : nf)++;
+    if (d.precision == LBM_PREC_FP64) {
+        return d.dim == 3 ? make_for_species<double, double, 3>(h, nf, nfix) : make_for_species<double, double, 2>(h, nf, nfix);
+    }
+    if (d.precision == LBM_PREC_FP32) {
+        return d.dim == 3 ? make_for_species<float, float, 3>(h, nf, nfix) : make_for_species<float, float, 2>(h, nf, nfix);
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+size_t lbm_struct_size(int which) { return which == 0 ? sizeof(lbm_desc) : (which == 1 ? sizeof(lbm_params) : 0); }
+
+int lbm_create(const lbm_desc* desc, lbm_handle** out) {
+    if (!desc || !out) return LBM_ERR_INVALID;
+    *out = nullptr;
+    if (desc->abi_version != LBM_ABI_VERSION) return LBM_ERR_INVALID;
+    if (desc->dim != 2 && desc->dim != 3) return LBM_ERR_INVALID;
+    if (desc->nx < 4 || desc->ny < 4 || (desc->dim == 3 && desc->nz < 4) || (desc->dim == 2 && desc->nz != 1))
+        return LBM_ERR_INVALID;
+    if (desc->n_species < 0 || desc->n_species > LBM_MAX_SPECIES) return LBM_ERR_INVALID;
+    lbm_handle* h = new (std::nothrow) lbm_handle();
+    if (!h) return LBM_ERR_OOM;
+    h->d = *desc;
+    h->err[0] = 0;
+    if (desc->collision != LBM_COLL_BGK) {
+        delete h;
+        return LBM_ERR_UNSUPPORTED;
+    }
+    h->impl.reset(make_impl(h));
+    if (!h->impl) {
+        delete h;
+        return LBM_ERR_UNSUPPORTED;
+    }
+    int rc = h->impl->alloc();
+    if (rc) {
+        fprintf(stderr, "lbm_create: %s\n", h->err);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_handle* h) { delete h; }
+
+int lbm_set_params(lbm_handle* h, const lbm_params* p) {
+    if (!h || !p) return LBM_ERR_INVALID;
+    if (p->n_reactions < 0 || p->n_reactions > LBM_MAX_REACTIONS)
+        return set_err(h, LBM_ERR_INVALID, "n_reactions out of range");
+    if (h->d.dim == 3 && p->bc_model != LBM_BCMODEL_NEE && p->bc_model != LBM_BCMODEL_NONE)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "only the NEE boundary model is implemented");
+    if (h->d.dim == 2 && p->bc_model != LBM_BCMODEL_NEE && p->bc_model != LBM_BCMODEL_NONE)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "only the NEE boundary model is implemented");
+    h->p = *p;
+    int rc = h->impl->set_params(*p);
+    if (rc == LBM_OK) h->params_set = 1;
+    return rc;
+}
+
+int lbm_set_field(lbm_handle* h, int field, int comp, const void* host, int dtype, size_t n) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    if (dtype != LBM_DTYPE_F64 && dtype != LBM_DTYPE_F32) return set_err(h, LBM_ERR_INVALID, "dtype %d", dtype);
+    return h->impl->set_field(field, comp, host, dtype, n);
+}
+
+int lbm_get_field(lbm_handle* h, int field, int comp, void* host, int dtype, size_t n) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    if (dtype != LBM_DTYPE_F64 && dtype != LBM_DTYPE_F32) return set_err(h, LBM_ERR_INVALID, "dtype %d", dtype);
+    return h->impl->get_field(field, comp, host, dtype, n);
+}
+
+int lbm_init(lbm_handle* h) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->params_set) return set_err(h, LBM_ERR_STATE, "lbm_set_params must precede lbm_init");
+    h->initialised = 0;
+    int rc = h->impl->init();
+    if (rc) return rc;
+    h->initialised = 1;
+    h->steps = 0;
+    return LBM_OK;
+}
+
+int lbm_step(lbm_handle* h, int nsteps) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->initialised) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_step");
+    if (nsteps < 0) return set_err(h, LBM_ERR_INVALID, "nsteps < 0");
+    if (nsteps == 0) return LBM_OK;
+    int rc = h->impl->step(nsteps);
+    if (rc) return rc;
+    if (h->p.check_divergence) {
+        double vmax = 0;
+        rc = h->impl->reduce(LBM_REDUCE_MAX_V, 0, &vmax);
+        if (rc) return rc;
+        if (!(vmax <= h->p.max_velocity))
+            return set_err(h, LBM_ERR_DIVERGED, "divergence after step %lld (max |u| = %g)", h->steps, vmax);
+    }
+    return LBM_OK;
+}
+
+int lbm_reduce(lbm_handle* h, int kind, int comp, double* out) {
+    if (!h || !out) return LBM_ERR_INVALID;
+    return h->impl->reduce(kind, comp, out);
+}
+
+int lbm_sync(lbm_handle* h) {
+    if (!h) return LBM_ERR_INVALID;
+    return h->impl->sync();
+}
+
+long long lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+int lbm_profile(lbm_handle* h, int enable) {
+    if (!h) return LBM_ERR_INVALID;
+    return h->impl->profile(enable);
+}
+
+int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* launches) {
+    if (!h || !total_ms || !launches) return LBM_ERR_INVALID;
+    return h->impl->profile_read(kernel, total_ms, launches);
+}
+
+const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null handle"; }
+
+}  // extern "C"

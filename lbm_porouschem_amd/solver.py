@@ -1,0 +1,999 @@
+"""Drop-in solver classes over the HIP C ABI.
+
+`LBM2DSolver` / `LBM3DSolver` keep the constructor, attribute names, setters,
+field protocol (`to_numpy` / `from_numpy` / `shape`), lifecycle (`init_simulation`,
+`step`, `tLattice`) and diagnostics (`get_max_v`, `get_min_T`, `get_max_T`,
+`log_info`) of the reference classes (src/LBM/LBM3D/__init__.py:12-61,
+src/LBM/LBM2D/__init__.py:30-70, setters in src/LBM/LBM3D/_io.py:121-489), so the
+reference's configuration interpreter can drive them unchanged.
+
+Fields are staged in host memory until `init_simulation()`; that call creates
+the device solver (liblbm_mi355x.so), uploads every field, and initialises the
+populations.  After that, macroscopic fields are read from the device; state
+fields that feed the populations can only be changed before initialisation.
+`step()` is lazy when no UpdateBCfunc callback is registered: steps are queued
+and launched as one batch the next time anything is observed.
+"""
+import numpy as np
+
+from . import _abi
+from .constant import MOLEMASS  # noqa: F401  (re-exported for configuration code)
+from .flag import (BC, BC_FLOW, BC_MODEL, CONDUCTIVITY_MODEL, DIFF_MODEL, FLUID_STATE_EQUATION, FORCE_TERM,
+                   PORO_MODEL, RADIATION_MODEL, SOURCE_TERM, SPECIE_UNIT, THERMAL_DIFF_MODEL, THERMO_MODEL,
+                   VISCOSITY_MODEL)
+
+_PREC = {"fp64": _abi.PREC_FP64, "fp32": _abi.PREC_FP32, "fp16": _abi.PREC_FP16}
+_COLL = {"bgk": _abi.COLL_BGK, "mrt": _abi.COLL_MRT}
+
+
+def _val(e):
+    return e.value if hasattr(e, "value") else int(e)
+
+
+class Field:
+    """A stand-in for a Taichi field: staged on the host before initialisation,
+    read from the device afterwards."""
+
+    def __init__(self, solver, fid, shape, ncomp=1, comp0=0, comp_stride=1, writable_after_init=True):
+        self._s = solver
+        self._fid = fid
+        self.shape = tuple(shape)
+        self._n = ncomp
+        self._comp0 = comp0
+        self._cstride = comp_stride
+        self._host = np.zeros(self.shape + ((ncomp,) if ncomp > 1 else ()), dtype=np.float64)
+        self._writable = writable_after_init
+
+    # component index for the C ABI
+    def _comp(self, a):
+        return self._comp0 + a * self._cstride
+
+    @property
+    def _size(self):
+        return int(np.prod(self.shape))
+
+    def to_numpy(self):
+        s = self._s
+        if s._handle is None:
+            out = self._host.copy()
+        else:
+            s._flush()
+            if self._n == 1:
+                out = s._lib.get_field(s._handle, self._fid, self._comp(0), self._size).reshape(self.shape)
+            else:
+                out = np.stack([s._lib.get_field(s._handle, self._fid, self._comp(a), self._size).reshape(self.shape)
+                                for a in range(self._n)], axis=-1)
+        return out.astype(s._np_dtype, copy=False)
+
+    def from_numpy(self, arr):
+        s = self._s
+        a = np.asarray(arr, dtype=np.float64)
+        want = self.shape + ((self._n,) if self._n > 1 else ())
+        if a.shape != want:
+            a = np.broadcast_to(a, want) if a.size in (1, int(np.prod(want))) and a.ndim <= len(want) else a
+            a = np.reshape(a, want)
+        if s._dtype_cast is not None:
+            a = a.astype(s._dtype_cast).astype(np.float64)
+        if s._handle is None:
+            self._host[...] = a
+            return
+        if not self._writable:
+            raise RuntimeError("this field feeds the populations: set it before init_simulation()")
+        s._flush()
+        if self._n == 1:
+            s._lib.set_field(s._handle, self._fid, self._comp(0), a)
+        else:
+            for c in range(self._n):
+                s._lib.set_field(s._handle, self._fid, self._comp(c), a[..., c])
+
+    def fill(self, value):
+        self.from_numpy(np.full(self.shape + ((self._n,) if self._n > 1 else ()), float(value)))
+
+    def __getitem__(self, idx):
+        return self.to_numpy()[idx]
+
+    def __setitem__(self, idx, value):
+        a = self.to_numpy().astype(np.float64)
+        a[idx] = value
+        self.from_numpy(a)
+
+    def _upload(self):
+        s = self._s
+        if self._n == 1:
+            s._lib.set_field(s._handle, self._fid, self._comp(0), self._host)
+        else:
+            for c in range(self._n):
+                s._lib.set_field(s._handle, self._fid, self._comp(c), self._host[..., c])
+
+    def _download(self):
+        s = self._s
+        if self._n == 1:
+            self._host[...] = s._lib.get_field(s._handle, self._fid, self._comp(0), self._size).reshape(self.shape)
+        else:
+            for c in range(self._n):
+                self._host[..., c] = s._lib.get_field(s._handle, self._fid, self._comp(c),
+                                                      self._size).reshape(self.shape)
+
+
+class SmallField:
+    """Host-only small field (v_BC, rho_BC, flow_BC, max_v): read into the
+    parameter block at each step."""
+
+    def __init__(self, n, ncomp=1):
+        self._a = np.zeros((n, ncomp) if ncomp > 1 else (n,), dtype=np.float64)
+
+    def __getitem__(self, i):
+        return self._a[i]
+
+    def __setitem__(self, i, v):
+        self._a[i] = v
+
+    def to_numpy(self):
+        return self._a.copy()
+
+    def from_numpy(self, a):
+        self._a[...] = a
+
+
+class ScalarField:
+    """Common part of TF/TS/species (LBM3D/_scalarField.py:5-141)."""
+
+    def __init__(self, name, solver, FIX=False):
+        self.name = name
+        self.LBM = solver
+        self.FIX = FIX
+        self.v_ref = 0.0
+        self.v_scale = 1.0
+        nf = solver._nfaces
+        self.BC = [BC.periodic] * nf
+        self.flux_BC = SmallField(nf)
+        self.s_BC = SmallField(nf)
+
+    def get_physical_value(self, v):
+        return v * self.v_scale + self.v_ref
+
+    def get_normalized_value(self, v_phys):
+        return (v_phys - self.v_ref) / self.v_scale
+
+    def set_BC(self, index_boundary, bc):
+        if not self.FIX:
+            self.BC[index_boundary] = bc
+
+    def set_BCs(self, BCs):
+        for i in range(self.LBM._nfaces):
+            self.set_BC(i, BCs[i])
+
+    def set_s_BC_value(self, index, s):
+        if not self.FIX:
+            self.s_BC[index] = self.get_normalized_value(s)
+
+    def set_s_BCs_value(self, s):
+        for i in range(self.LBM._nfaces):
+            self.set_s_BC_value(i, s[i])
+
+    def set_s_BC_flux(self, index, f):
+        self.flux_BC[index] = f
+
+    def set_s_BCs_flux(self, f):
+        for i in range(self.LBM._nfaces):
+            self.set_s_BC_flux(i, f[i])
+
+    def _bc_arrays(self):
+        t = [0] * 6
+        v = [0.0] * 6
+        for i in range(self.LBM._nfaces):
+            t[i] = _val(self.BC[i])
+            v[i] = float(self.s_BC[i])
+        return t, v
+
+
+class TemperatureFluid(ScalarField):
+    """Fluid temperature lattice (LBM3D/_thermal.py:6-106)."""
+
+    def __init__(self, name, solver):
+        super().__init__(name, solver)
+        self.thermal_diff_model = THERMAL_DIFF_MODEL.CONSTANT
+        self.Pr = 0.71
+        self.thermal_diff = 3e-5
+        self.conductivity_model = CONDUCTIVITY_MODEL.CONSTANT
+        self.cond = 1000
+        self.cond_poly = [0, 0, 0, 0, 0]
+        self.capacity_model = THERMO_MODEL.CONSTANT
+        self.cm = 1000
+        self.cm_poly = [0, 0, 0, 0, 0]
+        self.Trange = [0, 0, 0]
+        self.NASA_coef = [[0] * 7, [0] * 7]
+        self.S = solver._cell_field(_abi.FIELD_TF, writable=False)
+        self.dS = solver._cell_field(_abi.FIELD_TF_DS, writable=False)
+
+    def physical_value(self, v):
+        return self.get_physical_value(v)
+
+
+class TemperatureSolid(ScalarField):
+    """Solid temperature lattice (LBM3D/_thermal.py:109-221)."""
+
+    def __init__(self, name, solver, isRadiation=False):
+        super().__init__(name, solver)
+        self.exchangeCoef = solver._cell_field(_abi.FIELD_EXCH_COEF, writable=False)
+        self.exchangeSurface = solver._cell_field(_abi.FIELD_EXCH_SURF, writable=False)
+        self.rhoSolid = Field(solver, -1, solver._cell_shape)  # declared by the reference, never read by the step
+        if isRadiation:
+            self.Tambient = 300.0
+            self.radiation_model = RADIATION_MODEL.NONE
+            self.radiation_surface = solver._cell_field(_abi.FIELD_RAD_SURF, writable=False)
+            self.emissivity = solver._cell_field(_abi.FIELD_EMISSIVITY, writable=False)
+        self.thermal_diff_model = THERMAL_DIFF_MODEL.CONSTANT
+        self.thermal_diff = 3e-5
+        self.Pr = 0.71
+        self.conductivity_model = CONDUCTIVITY_MODEL.CONSTANT
+        self.cond = 0.2
+        self.cond_poly = [0.0] * 5
+        self.capacity_model = THERMO_MODEL.CONSTANT
+        self.cm = 1000.0
+        self.cm_poly = [0.0] * 5
+        self.Trange = [0.0, 0, 0]
+        self.NASA_coef = [[0.0] * 7, [0.0] * 7]
+        self.S = solver._cell_field(_abi.FIELD_TS, writable=False)
+        self.dS = solver._cell_field(_abi.FIELD_TS_DS, writable=False)
+
+    def physical_value(self, v):
+        return self.get_physical_value(v)
+
+
+class Specie(ScalarField):
+    """Species mass-fraction (fluid) or density (FIX) field (LBM3D/_chemical.py:7-137)."""
+
+    def __init__(self, name, solver, FIX=False, Mmass=1.0):
+        super().__init__(name, solver, FIX)
+        self.molemass = Mmass / 1000
+        if not FIX:
+            self.viscosity_type = VISCOSITY_MODEL.NONE
+            self.visco = 1e-5
+            self.coefSutherland = [0.0, 0.0]
+            self.diff_model = DIFF_MODEL.CONSANT
+            self.diff = 1e-5
+        self.thermo_model = THERMO_MODEL.CONSTANT
+        self.enthalpy = 100.0
+        self.capa = 100.0
+        self.Trange = [0.0, 0.0, 0.0]
+        self.NASAcoef = [[0.0] * 7] * 2
+        self.Sc = 1.0
+        self.cond_model = CONDUCTIVITY_MODEL.CONSTANT
+        self.cond = 1.0
+        self.cond_poly = [0, 0, 0, 0, 0]
+        idx = len(solver.species)
+        self.S = solver._cell_field(_abi.FIELD_SPECIES, comp0=idx, writable=False)
+        self.dS = solver._cell_field(_abi.FIELD_SPECIES_DS, comp0=idx, writable=False)
+
+    def __str__(self):
+        return f"{self.name} : {'SOLID' if self.FIX else 'FLUID'}, molar mass {self.molemass} kg/mol"
+
+    __repr__ = __str__
+
+
+class Reaction:
+    """One Arrhenius reaction; stoichiometry parsed like LBM3D/_chemical.py:158-183."""
+
+    def __init__(self, lb, formula, A, Ea, b=0, Tmin=0, deltaH=0, name="unnamed reaction", unit=SPECIE_UNIT.MASS,
+                 fixDH=True):
+        self.formula = formula
+        self.name = name
+        self.A, self.b, self.Ea, self.Tmin, self.deltaH = A, b, Ea, Tmin, deltaH
+        self.LBM = lb
+        self.unit = unit
+        self.isFixDH = fixDH
+        self.specieNum = len(lb.species)
+        self.coefProduct = [0.0] * self.specieNum
+        self.coefReactant = [0.0] * self.specieNum
+        self.coefRate = [0.0] * self.specieNum
+        self.parse_formula(formula)
+
+    def parse_formula(self, formula):
+        formula = formula.replace("<=>", "|").replace("=>", "|")
+        reactant, product = formula.split("|")
+        for side, is_reactant in ((reactant, True), (product, False)):
+            for term in [t.strip() for t in side.split("+")]:
+                parts = term.split()
+                coef, specie = (float(parts[0]), parts[1]) if len(parts) > 1 else (1, parts[0])
+                j = self.LBM.specieName.index(specie)
+                if is_reactant:
+                    self.coefReactant[j] = coef
+                    self.coefRate[j] = coef
+                else:
+                    self.coefProduct[j] = coef
+
+    def __str__(self):
+        return self.formula
+
+    __repr__ = __str__
+
+
+class Reactions:
+    def __init__(self, lbm):
+        self.LBM = lbm
+        self.reactions = []
+
+    def add_reaction(self, reaction):
+        self.reactions.append(reaction)
+
+    def __str__(self):
+        d = "Reactions:\n"
+        for r in self.reactions:
+            d += f" - {r.name}\n   {r}\n"
+        return d + f"{len(self.reactions)} reaction(s) in total.\n"
+
+
+class LBMSolverBase:
+    """State, setters and lifecycle shared by the 2D and 3D solvers."""
+
+    _dim = 3
+
+    def __init__(self, X, Y, Z, dx=0.001, dt=0.001, name="LBM", isThermal=False, isChemical=False, isPoro=False,
+                 isRadiation=False, precision="fp32", collision="bgk", device=0, lib=None):
+        self.name = name
+        self.t = 0.0
+        self.tol = 1e-6
+        self.X, self.Y = X, Y
+        self.tLattice = 0
+        self.dx, self.dt = dx, dt
+        self.nx = int(self.X / self.dx)
+        self.ny = int(self.Y / self.dx)
+        if self._dim == 3:
+            self.Z = Z
+            self.nz = int(self.Z / self.dx)
+        else:
+            self.Z = dx
+            self.nz = 1
+        self._nfaces = 2 * self._dim
+        self._cell_shape = (self.nx, self.ny, self.nz)
+        self.precision = precision
+        self.collision = collision
+        self._prec = _PREC[precision]
+        self._np_dtype = np.float64 if precision == "fp64" else np.float32
+        self._dtype_cast = None if precision == "fp64" else np.float32
+        self._device = device
+        self._lib = lib if lib is not None else _abi.load_library()
+        self._handle = None
+        self._pending = 0
+        self._sent_params = None
+        self.viscosity_model = VISCOSITY_MODEL.NONE
+        self.visco = 2e-5
+        self.sutherland_coef = [1.6e-6, 170]
+        self.source_term_model = SOURCE_TERM.NONE
+        self.force_term_model = FORCE_TERM.NONE
+        self.boundary_condition_model = BC_MODEL.NEE
+        self.EOS = FLUID_STATE_EQUATION.IDEAL_GAS
+        self.max_velocity = 0.0  # > 0 enables the divergence detector of lbm_step
+        nf = self._nfaces
+        self.rho = self._cell_field(_abi.FIELD_RHO, writable=False)
+        self.v = self._cell_field(_abi.FIELD_VEL, ncomp=3, writable=False)
+        self.solid = self._cell_field(_abi.FIELD_SOLID, writable=False)
+        self.bc = [BC_FLOW.periodic] * nf
+        self.bc_v = [BC.periodic] * nf
+        self.bc_rho = [BC.periodic] * nf
+        self.v_BC = SmallField(nf, 3)
+        self.rho_BC = SmallField(nf)
+        self.flow_BC = SmallField(nf)
+        self.max_v = SmallField(1)
+        self.v_bc_profile = [Field(self, _abi.FIELD_V_BC_PROFILE, self._face_shape(f), ncomp=3, comp0=3 * f)
+                             for f in range(nf)]
+        self.rho_bc_profile = [Field(self, _abi.FIELD_RHO_BC_PROFILE, self._face_shape(f), comp0=f)
+                               for f in range(nf)]
+        self.sideName = ["left", "right", "front", "back", "bottom", "top"][:nf] if self._dim == 3 else \
+            ["left", "right", "bottom", "top"]
+        self.UpdateBCfunc = []
+        self.GetVariableFunc = []
+        self.TEMPERATURE = isThermal
+        self.CHEMISTRY = isChemical
+        self.PORO = isPoro
+        self.RADIATION = isRadiation and isThermal
+        if self.TEMPERATURE:
+            self.TF = TemperatureFluid("Temperature of Fluid", self)
+            self.TS = TemperatureSolid("Temperature of Solid", self, isRadiation=self.RADIATION)
+            self.rhos = self._cell_field(_abi.FIELD_RHOS, writable=False)
+            self.min_T = SmallField(1)
+            self.max_T = SmallField(1)
+        if self.CHEMISTRY:
+            self.specieName = []
+            self.species = []
+            self.reactions = Reactions(self)
+        if self.PORO:
+            self.poro_model = PORO_MODEL.SPHERICAL
+            self.coefDarcy = self._cell_field(_abi.FIELD_COEF_DARCY, writable=False)
+            self.coefForchheimer = self._cell_field(_abi.FIELD_COEF_FORCH, writable=False)
+        self.default_init()
+        self.exportPath = "vtk"
+        self.snapshotPath = "snapshot.yaml"
+
+    # ------------------------------------------------------------------
+    def _face_shape(self, f):
+        if f < 2:
+            return (1, self.ny, self.nz)
+        if f < 4:
+            return (self.nx, 1, self.nz)
+        return (self.nx, self.ny, 1)
+
+    def _cell_field(self, fid, ncomp=1, comp0=0, writable=True):
+        return Field(self, fid, self._cell_shape, ncomp=ncomp, comp0=comp0, writable_after_init=writable)
+
+    def default_init(self):
+        # LBM3D/_initialize.py:8-16 (rho_BC = 0 in 3D), LBM2D/_initialize.py:7-15 (rho_BC = 1 in 2D)
+        for i in range(self._nfaces):
+            self.rho_BC[i] = 0.0 if self._dim == 3 else 1.0
+            self.v_BC[i] = [0, 0, 0]
+        self.rho._host[...] = 1.0
+
+    def __del__(self):
+        try:
+            if self._handle is not None:
+                self._lib.destroy(self._handle)
+                self._handle = None
+        except Exception:
+            pass
+
+    def __str__(self):
+        return (f"LBM{self._dim}DSolver: {self.name}, Size: {self.nx} x {self.ny} x {self.nz}, dx: {self.dx}, "
+                f"dt: {self.dt}, Thermal: {self.TEMPERATURE}, Chemical: {self.CHEMISTRY}, Poro: {self.PORO}, "
+                f"Radiation: {self.RADIATION}")
+
+    __repr__ = __str__
+
+    def description(self):
+        return "\n" + str(self) + "\n"
+
+    # ------------------------------------------------------------------
+    # fields (LBM3D/_io.py:37-141)
+    def init_field(self, field, param):
+        if type(param) in [float, int]:
+            data = float(param) * np.ones(shape=field.shape, dtype=np.float32)
+            field.from_numpy(data)
+        if type(param) is str:
+            in_dat = np.loadtxt(param, dtype=np.float32)
+            field.from_numpy(np.reshape(in_dat, field.shape, order="F"))
+        if type(param) == np.ndarray:
+            field.from_numpy(param)
+
+    def init_field3(self, field, param1, param2, param3):
+        def one(p):
+            if type(p) in [float, int]:
+                return p * np.ones(shape=self._cell_shape)
+            if type(p) is str:
+                return np.reshape(np.loadtxt(p), self._cell_shape, order="F")
+            return p
+        field.from_numpy(np.stack([one(param1), one(param2), one(param3)], axis=-1))
+
+    def add_solid(self, solid):
+        s = self.solid.to_numpy().astype(np.float64)
+        s += solid
+        self.init_field(self.solid, s)
+
+    def add_rho_solid(self, rhos):
+        s = self.rhos.to_numpy().astype(np.float64)
+        s += rhos
+        self.init_field(self.rhos, s)
+
+    def set_heat_exchange_surface(self, surface):
+        self.init_field(self.TS.exchangeSurface, surface)
+
+    def set_heat_exchange_coef(self, coef):
+        self.init_field(self.TS.exchangeCoef, coef)
+
+    # flow
+    def set_viscosity(self, niu):
+        self.visco = niu
+        self.viscosity_model = VISCOSITY_MODEL.CONSTANT
+
+    def set_viscosity_sutherland(self, As, Ts):
+        self.sutherland_coef = [As, Ts]
+        self.viscosity_model = VISCOSITY_MODEL.SUTHERLAND
+
+    def set_viscosity_mixture(self):
+        self.viscosity_model = VISCOSITY_MODEL.MIXTURE
+
+    # porous media
+    def set_poro_Darcy(self, s, coefDarcy):
+        self.poro_model = PORO_MODEL.DARCY
+        coefDarcy *= self.dx ** 2
+        self.init_field(self.coefDarcy, coefDarcy * s)
+
+    def set_poro_Darcy_Forchheimer(self, s, coefDarcy, coefForchheimer):
+        self.poro_model = PORO_MODEL.DARCYFORCHHEIMER
+        coefDarcy *= self.dx ** 2
+        coefForchheimer *= self.dx
+        self.init_field(self.coefDarcy, coefDarcy * s)
+        self.init_field(self.coefForchheimer, coefForchheimer * s)
+
+    # fluid temperature
+    def set_fluid_thermal_diff(self, diff):
+        self.TF.thermal_diff_model = THERMAL_DIFF_MODEL.CONSTANT
+        self.TF.thermal_diff = diff
+
+    def set_fluid_Prandtl(self, Pr):
+        self.TF.thermal_diff_model = THERMAL_DIFF_MODEL.PRANDTL
+        self.TF.Pr = Pr
+
+    def set_fluid_thermal_diff_derived(self):
+        self.TF.thermal_diff_model = THERMAL_DIFF_MODEL.DERIVED
+
+    def set_fluid_conductivity(self, value):
+        self.TF.conductivity_model = CONDUCTIVITY_MODEL.CONSTANT
+        self.TF.cond = value
+
+    def set_fluid_conductivity_poly(self, poly):
+        self.TF.conductivity_model = CONDUCTIVITY_MODEL.POLYNOMIAL
+        self.TF.cond_poly = poly
+
+    def set_fluid_conductivity_mixture(self):
+        self.TF.conductivity_model = CONDUCTIVITY_MODEL.MIXTURE
+
+    def set_fluid_capacity(self, value):
+        self.TF.capacity_model = THERMO_MODEL.CONSTANT
+        self.TF.cm = value
+
+    def set_fluid_capacity_poly(self, poly):
+        self.TF.capacity_model = THERMO_MODEL.POLYNOMIAL
+        self.TF.cm_poly = poly
+
+    def set_fluid_capacity_NASA7(self, Trange, data):
+        self.TF.capacity_model = THERMO_MODEL.NASA7
+        self.TF.Trange = Trange
+        self.TF.NASA_coef = data
+
+    def set_fluid_capacity_mixture(self):
+        self.TF.capacity_model = THERMO_MODEL.MIXTURE
+
+    def set_fluid_Trange(self, Trange):
+        self.TF.v_ref = Trange[0]
+        self.TF.v_scale = Trange[1] - Trange[0]
+
+    # solid temperature
+    def set_solid_thermal_diff(self, diff):
+        self.TS.thermal_diff_model = THERMAL_DIFF_MODEL.CONSTANT
+        self.TS.thermal_diff = diff
+
+    def set_solid_thermal_diff_derived(self):
+        self.TS.thermal_diff_model = THERMAL_DIFF_MODEL.DERIVED
+
+    def set_solid_conductivity(self, value):
+        self.TS.conductivity_model = CONDUCTIVITY_MODEL.CONSTANT
+        self.TS.cond = value
+
+    def set_solid_conductivity_poly(self, poly):
+        self.TS.conductivity_model = CONDUCTIVITY_MODEL.POLYNOMIAL
+        self.TS.cond_poly = poly
+
+    def set_solid_conductivity_mixture(self):
+        self.TS.conductivity_model = CONDUCTIVITY_MODEL.MIXTURE
+
+    def set_solid_capacity(self, value):
+        self.TS.capacity_model = THERMO_MODEL.CONSTANT
+        self.TS.cm = value
+
+    def set_solid_capacity_poly(self, poly):
+        self.TS.capacity_model = THERMO_MODEL.POLYNOMIAL
+        self.TS.cm_poly = poly
+
+    def set_solid_capacity_NASA7(self, Trange, data):
+        self.TS.capacity_model = THERMO_MODEL.NASA7
+        self.TS.Trange = Trange
+        self.TS.NASA_coef = data
+
+    def set_solid_capacity_mixture(self):
+        self.TS.capacity_model = THERMO_MODEL.MIXTURE
+
+    def set_solid_Trange(self, Trange):
+        self.TS.v_ref = Trange[0]
+        self.TS.v_scale = Trange[1] - Trange[0]
+
+    def set_radiation(self, model, param):
+        if model == RADIATION_MODEL.SURFACE_UNIFORM:
+            self.TS.radiation_model = model
+            self.TS.Tambient = float(param)
+
+    # species
+    def set_specie(self, specie, Fix=False, molemass=1.0):
+        self.species.append(Specie(specie, self, Mmass=molemass, FIX=Fix))
+        self.specieName.append(specie)
+
+    def _sp(self, name):
+        return self.species[self.specieName.index(name)]
+
+    def set_specie_viscosity(self, specieName, value):
+        s = self._sp(specieName)
+        s.viscosity_type = VISCOSITY_MODEL.CONSTANT
+        s.visco = value
+
+    def set_specie_viscosity_sutherland(self, specieName, coef):
+        s = self._sp(specieName)
+        s.viscosity_type = VISCOSITY_MODEL.SUTHERLAND
+        s.coefSutherland = coef
+
+    def set_specie_NASA7(self, specieName, TRange, coef):
+        s = self._sp(specieName)
+        s.Trange = TRange
+        s.NASAcoef = coef
+
+    def set_species(self, species, FIX=None, molemass=None):
+        if FIX is None:
+            FIX = [False] * len(species)
+        for i, name in enumerate(species):
+            self.set_specie(name, FIX[i])
+
+    def init_specie(self, name, param):
+        self.init_field(self._sp(name).S, param)
+
+    def set_specie_diff(self, specieName, diff):
+        s = self._sp(specieName)
+        s.diff_model = DIFF_MODEL.CONSANT
+        s.diff = diff
+
+    def set_specie_diff_Schmidt(self, specieName, Sc):
+        s = self._sp(specieName)
+        s.diff_model = DIFF_MODEL.SCHMIDT
+        s.Sc = Sc
+
+    def set_specie_enthalpy(self, specieName, enthalpy):
+        s = self._sp(specieName)
+        s.thermo_model = THERMO_MODEL.CONSTANT
+        s.enthalpy = enthalpy
+
+    def set_specie_capacity(self, specieName, capacity):
+        s = self._sp(specieName)
+        s.thermo_model = THERMO_MODEL.CONSTANT
+        s.capa = capacity
+
+    def set_specie_conductivity(self, name, lamb):
+        s = self._sp(name)
+        s.cond_model = CONDUCTIVITY_MODEL.CONSTANT
+        s.cond = lamb
+
+    def set_specie_conductivity_poly(self, name, poly):
+        s = self._sp(name)
+        s.cond_model = CONDUCTIVITY_MODEL.POLYNOMIAL
+        s.cond_poly = list(poly)
+
+    def add_reaction(self, formula, A, Ea, b=0, Tmin=0, deltaH=0, name="unnamed", unit=SPECIE_UNIT.MASS, fixDH=True):
+        self.reactions.add_reaction(Reaction(self, formula, A, Ea, b, Tmin, deltaH, name, unit, fixDH))
+
+    # boundary conditions (LBM3D/_io.py:385-489)
+    def set_BC(self, i, bc):
+        self.bc[i] = bc
+        if bc == BC_FLOW.inlet:
+            self.set_v_BC(i, BC.fixedValue)
+            self.set_rho_BC(i, BC.zeroGradient)
+        if bc == BC_FLOW.outlet:
+            self.set_v_BC(i, BC.zeroGradient)
+            self.set_rho_BC(i, BC.fixedValue)
+        if bc == BC_FLOW.symmetric:
+            self.set_v_BC(i, BC.zeroGradient)
+            self.set_rho_BC(i, BC.zeroGradient)
+        if bc == BC_FLOW.wall:
+            self.set_v_BC(i, BC.fixedValue)
+            self.set_rho_BC(i, BC.zeroGradient)
+        if bc == BC_FLOW.inlet_flow:
+            self.set_v_BC(i, BC.fixedValue)
+            self.set_rho_BC(i, BC.zeroGradient)
+
+    def set_BCs(self, BCs):
+        for i in range(self._nfaces):
+            self.set_BC(i, BCs[i])
+
+    def set_v_BC(self, i, bc):
+        self.bc_v[i] = bc
+
+    def set_rho_BC(self, i, bc):
+        self.bc_rho[i] = bc
+
+    def set_v_BC_value(self, i, v):
+        self.v_BC[i] = v
+
+    def set_v_BCs_value(self, vs):
+        for i in range(self._nfaces):
+            self.set_v_BC_value(i, vs[i])
+
+    def set_rho_BC_value(self, i, r):
+        self.rho_BC[i] = r
+
+    def set_rho_BCs_value(self, rhos):
+        for i in range(self._nfaces):
+            self.set_rho_BC_value(i, rhos[i])
+
+    def set_flow_BC_value(self, i, f):
+        self.flow_BC[i] = f
+
+    def set_flow_BCs_value(self, fs):
+        for i in range(4):
+            self.set_flow_BC_value(i, fs[i])
+
+    def set_TS_BC(self, i, bc):
+        self.TS.set_BC(i, bc)
+
+    def set_TS_BCs(self, BCs):
+        self.TS.set_BCs(BCs)
+
+    def set_TS_BC_value(self, i, T):
+        self.TS.set_s_BC_value(i, T)
+
+    def set_TS_BCs_value(self, Ts):
+        self.TS.set_s_BCs_value(Ts)
+
+    def set_TF_BC(self, i, bc):
+        self.TF.set_BC(i, bc)
+
+    def set_TF_BCs(self, BCs):
+        self.TF.set_BCs(BCs)
+
+    def set_TF_BC_value(self, i, T):
+        self.TF.set_s_BC_value(i, T)
+
+    def set_TF_BCs_value(self, Ts):
+        self.TF.set_s_BCs_value(Ts)
+
+    def set_specie_BC(self, specie, i, bc):
+        self._sp(specie).set_BC(i, bc)
+
+    def set_specie_BCs(self, specie, BCs):
+        self._sp(specie).set_BCs(BCs)
+
+    def set_species_BC(self, i, bc):
+        for s in self.species:
+            s.set_BC(i, bc)
+
+    def set_species_BCs(self, BCs):
+        for s in self.species:
+            s.set_BCs(BCs)
+
+    def set_specie_BC_value(self, specie, i, S):
+        self._sp(specie).set_s_BC_value(i, S)
+
+    def set_specie_BCs_value(self, specie, Ss):
+        self._sp(specie).set_s_BCs_value(Ss)
+
+    def set_specie_BC_flux(self, specie, i, f):
+        self._sp(specie).set_s_BC_flux(i, f)
+
+    def set_specie_BCs_flux(self, specie, fs):
+        self._sp(specie).set_s_BCs_flux(fs)
+
+    def load_stl(self, *args, **kwargs):
+        raise NotImplementedError("STL voxelisation is not part of this package; pass a solid array to add_solid()")
+
+    # ------------------------------------------------------------------
+    # device lifecycle
+    def _all_fields(self):
+        fl = [self.rho, self.v, self.solid]
+        fl += self.v_bc_profile + self.rho_bc_profile
+        if self.TEMPERATURE:
+            fl += [self.TF.S, self.TF.dS, self.TS.S, self.TS.dS, self.rhos, self.TS.exchangeCoef,
+                   self.TS.exchangeSurface]
+            if self.RADIATION:
+                fl += [self.TS.radiation_surface, self.TS.emissivity]
+        if self.CHEMISTRY:
+            for s in self.species:
+                fl += [s.S, s.dS]
+        if self.PORO:
+            fl += [self.coefDarcy, self.coefForchheimer]
+        return fl
+
+    def _desc(self):
+        d = _abi.LbmDesc()
+        d.abi_version = _abi.ABI_VERSION
+        d.dim = self._dim
+        d.nx, d.ny, d.nz = self.nx, self.ny, self.nz
+        d.thermal = int(bool(self.TEMPERATURE))
+        d.chemical = int(bool(self.CHEMISTRY))
+        d.poro = int(bool(self.PORO))
+        d.radiation = int(bool(self.RADIATION))
+        if self.CHEMISTRY:
+            if len(self.species) > _abi.MAX_SPECIES:
+                raise ValueError(f"at most {_abi.MAX_SPECIES} species are supported")
+            d.n_species = len(self.species)
+            for j, s in enumerate(self.species):
+                d.species_fix[j] = int(bool(s.FIX))
+        d.precision = self._prec
+        d.collision = _COLL[self.collision]
+        d.device = self._device
+        return d
+
+    def _params(self):
+        p = _abi.LbmParams()
+        p.dx, p.dt, p.tol = float(self.dx), float(self.dt), float(self.tol)
+        p.eos = _val(self.EOS)
+        p.visc_model = _val(self.viscosity_model)
+        p.visco = float(self.visco)
+        p.sutherland[0], p.sutherland[1] = map(float, self.sutherland_coef)
+        p.force_term = _val(self.force_term_model)
+        p.bc_model = _val(self.boundary_condition_model)
+        p.poro_model = _val(self.poro_model) if self.PORO else 0
+        for i in range(self._nfaces):
+            p.bc_flow[i] = _val(self.bc[i])
+            p.bc_v[i] = _val(self.bc_v[i])
+            p.bc_rho[i] = _val(self.bc_rho[i])
+            vb = np.asarray(self.v_BC[i], dtype=np.float64).reshape(-1)
+            for a in range(3):
+                p.v_bc[i][a] = float(vb[a])
+            p.rho_bc[i] = float(self.rho_BC[i])
+            p.flow_bc[i] = float(self.flow_BC[i])
+        if self.TEMPERATURE:
+            for dst, src in ((p.tf, self.TF), (p.ts, self.TS)):
+                dst.v_ref, dst.v_scale = float(src.v_ref), float(src.v_scale)
+                dst.diff_model = _val(src.thermal_diff_model)
+                dst.diff = float(src.thermal_diff)
+                dst.Pr = float(src.Pr)
+                dst.cond_model = _val(src.conductivity_model)
+                dst.cond = float(src.cond)
+                for k, c in enumerate(list(src.cond_poly)[:5]):
+                    dst.cond_poly[k] = float(c)
+                dst.cap_model = _val(src.capacity_model)
+                dst.cm = float(src.cm)
+                for k, c in enumerate(list(src.cm_poly)[:5]):
+                    dst.cm_poly[k] = float(c)
+                for k, c in enumerate(list(src.Trange)[:3]):
+                    dst.Trange[k] = float(c)
+                for r in range(2):
+                    for k, c in enumerate(list(src.NASA_coef[r])[:7]):
+                        dst.nasa[r][k] = float(c)
+                t, v = src._bc_arrays()
+                for f in range(6):
+                    dst.bc_type[f] = t[f]
+                    dst.bc_value[f] = v[f]
+            if self.RADIATION:
+                p.radiation_model = _val(self.TS.radiation_model)
+                p.t_ambient = float(self.TS.Tambient)
+        if self.CHEMISTRY:
+            for j, s in enumerate(self.species):
+                q = p.species[j]
+                q.molemass = float(s.molemass)
+                if not s.FIX:
+                    q.visc_model = _val(s.viscosity_type)
+                    q.visco = float(s.visco)
+                    q.sutherland[0], q.sutherland[1] = map(float, s.coefSutherland)
+                    q.diff_model = _val(s.diff_model)
+                    q.diff = float(s.diff)
+                q.Sc = float(s.Sc)
+                q.thermo_model = _val(s.thermo_model)
+                q.enthalpy = float(s.enthalpy)
+                q.capa = float(s.capa)
+                q.cond_model = _val(s.cond_model)
+                q.cond = float(s.cond)
+                for k, c in enumerate(list(s.cond_poly)[:5]):
+                    q.cond_poly[k] = float(c)
+                t, v = s._bc_arrays()
+                for f in range(6):
+                    q.bc_type[f] = t[f]
+                    q.bc_value[f] = v[f]
+            rl = self.reactions.reactions
+            if len(rl) > _abi.MAX_REACTIONS:
+                raise ValueError(f"at most {_abi.MAX_REACTIONS} reactions are supported")
+            p.n_reactions = len(rl)
+            for r, rx in enumerate(rl):
+                q = p.reactions[r]
+                q.A, q.Ea, q.b, q.Tmin = float(rx.A), float(rx.Ea), float(rx.b), float(rx.Tmin)
+                for j in range(len(self.species)):
+                    q.coef_reactant[j] = float(rx.coefReactant[j]) if j < len(rx.coefReactant) else 0.0
+                    q.coef_product[j] = float(rx.coefProduct[j]) if j < len(rx.coefProduct) else 0.0
+                    q.coef_rate[j] = float(rx.coefRate[j]) if j < len(rx.coefRate) else 0.0
+        if self.max_velocity > 0:
+            p.check_divergence = 1
+            p.max_velocity = float(self.max_velocity)
+        return p
+
+    def _push_params(self):
+        p = self._params()
+        raw = bytes(p)
+        if raw != self._sent_params:
+            self._lib.set_params(self._handle, p)
+            self._sent_params = raw
+
+    def init_simulation(self):
+        """Create the device solver, upload every staged field, initialise the populations
+        (LBM3D/_initialize.py:17-88)."""
+        if self._handle is not None:
+            self._flush()
+            for f in self._all_fields():
+                f._download()
+            self._lib.destroy(self._handle)
+            self._handle = None
+        h = self._lib.create(self._desc())
+        self._handle = h
+        self._sent_params = None
+        try:
+            for f in self._all_fields():
+                f._upload()
+            self._push_params()
+            self._lib.init(h)
+        except Exception:
+            self._lib.destroy(h)
+            self._handle = None
+            raise
+
+    def _flush(self):
+        if self._handle is not None and self._pending:
+            n = self._pending
+            self._pending = 0
+            self._push_params()
+            self._lib.step(self._handle, n)
+
+    def updateBC(self, t):
+        for func in self.UpdateBCfunc:
+            func(self, t)
+
+    def step(self):
+        """One time step (LBM3D/_evolution.py:11-14)."""
+        if self._handle is None:
+            raise RuntimeError("init_simulation() must be called before step()")
+        if self.UpdateBCfunc:
+            self._flush()
+            self.updateBC(self.t)
+            self._push_params()
+            self._lib.step(self._handle, 1)
+        else:
+            self._pending += 1
+        self.tLattice += 1
+
+    def run(self, nsteps):
+        """Advance nsteps in one batch of launches."""
+        for _ in range(int(nsteps)):
+            self.step()
+        self._flush()
+
+    def sync(self):
+        self._flush()
+        if self._handle is not None:
+            self._lib.sync(self._handle)
+
+    # diagnostics (LBM3D/_io.py:542-592)
+    def get_max_v(self):
+        self._flush()
+        self.max_v[0] = self._lib.reduce(self._handle, _abi.REDUCE_MAX_V)
+        return self.max_v[0]
+
+    def get_max_T(self):
+        if not self.TEMPERATURE:
+            return -1
+        self._flush()
+        self.max_T[0] = self._lib.reduce(self._handle, _abi.REDUCE_MAX_TF)
+        return self.TF.get_physical_value(self.max_T[0])
+
+    def get_min_T(self):
+        if not self.TEMPERATURE:
+            return -1
+        self._flush()
+        self.min_T[0] = self._lib.reduce(self._handle, _abi.REDUCE_MIN_TF)
+        return self.min_T[0]
+
+    def reduce(self, kind, comp=0):
+        """FP64 balances (mass, temperature sums, FIX-species integral) of the current state."""
+        self._flush()
+        return self._lib.reduce(self._handle, kind, comp)
+
+    def log_info(self):
+        p = f"    t(LU)={self.tLattice} : Max velocity magnitude (LU) : {self.get_max_v():.7f}, "
+        if self.TEMPERATURE:
+            p += f"Min temperature: {self.get_min_T():.7f} K, Max temperature : {self.get_max_T():.7f}"
+        return p
+
+    def export_snapshot(self, config):
+        """The reference's export_snapshot is a no-op (LBM3D/_io.py:593-595)."""
+        pass
+
+
+class LBM3DSolver(LBMSolverBase):
+    """D3Q19 flow + D3Q7 scalar lattices (reference LBM3DSolver, LBM3D/__init__.py:12-61)."""
+
+    _dim = 3
+
+    def __init__(self, X, Y, Z, dx=0.001, dt=0.001, name="default_LBM", isThermal=False, isChemical=False,
+                 isPoro=False, isRadiation=False, **kw):
+        super().__init__(X, Y, Z, dx, dt, name, isThermal, isChemical, isPoro, isRadiation, **kw)
+
+
+class LBM2DSolver(LBMSolverBase):
+    """D2Q9 flow + D2Q5 scalar lattices (reference LBM2DSolver, LBM2D/__init__.py:30-70)."""
+
+    _dim = 2
+
+    def __init__(self, X, Y, Z=None, dx=0.001, dt=0.001, name="default_LBM", isThermal=False, isChemical=False,
+                 isPoro=False, isRadiation=False, **kw):
+        super().__init__(X, Y, Z, dx, dt, name, isThermal, isChemical, isPoro, isRadiation, **kw)

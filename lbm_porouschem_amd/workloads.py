@@ -1,0 +1,174 @@
+"""Seeded synthetic workloads of the coupled step.
+
+`plain2d` is the 2D case (256x128, FP64, D2Q9 flow + D2Q5 fluid and solid
+temperature, gray porous disks); `coupled3d` is the 3D coupled case (256^3,
+FP32, D3Q19 flow, D3Q7 fluid/solid temperature and 4 fluid species, 2 FIX
+species, three competing Arrhenius pyrolysis reactions, gray porous spheres).
+Both return a configured solver ready for init_simulation(); the same builder
+drives the HIP library or, in the tests, the CPU checker (``lib=``).
+
+The physical units follow the reference's convention (lattice constants
+derived from dx, dt: viscosity * dt / dx^2, LBM3D/_io.py:121-171): dx = dt = 1e-4
+(m, s) so a kinematic viscosity of 1e-5 m^2/s gives tau = 0.8.
+"""
+import numpy as np
+
+from .flag import BC, BC_FLOW, FORCE_TERM, SOURCE_TERM
+
+R_GAS = 8.314441
+
+
+def gray_packing(shape, r_lo, r_hi, coverage, seed, solid_value):
+    """Overlapping random disks (2D, shape[2] == 1) or spheres placed until the
+    covered fraction reaches `coverage`; covered cells get `solid_value`.
+    Returns (solid fraction array, boolean mask)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    dim = 2 if nz == 1 else 3
+    mask = np.zeros(shape, dtype=bool)
+    target = coverage * mask.size
+    covered = 0
+    while covered < target:
+        r = rng.uniform(r_lo, r_hi)
+        c = rng.uniform(0, 1, size=3) * np.array([nx, ny, nz])
+        lo = np.maximum(np.floor(c - r).astype(int), 0)
+        hi = np.minimum(np.ceil(c + r).astype(int) + 1, [nx, ny, nz])
+        if dim == 2:
+            lo[2], hi[2], c[2] = 0, 1, 0.0
+        xs = np.arange(lo[0], hi[0])[:, None, None]
+        ys = np.arange(lo[1], hi[1])[None, :, None]
+        zs = np.arange(lo[2], hi[2])[None, None, :]
+        inside = (xs - c[0]) ** 2 + (ys - c[1]) ** 2 + ((zs - c[2]) ** 2 if dim == 3 else 0) <= r * r
+        sub = mask[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        covered += int(np.count_nonzero(inside & ~sub))
+        sub |= inside
+    return mask * float(solid_value), mask
+
+
+def plain2d(lib=None, nx=256, ny=128, seed=2024, precision="fp64"):
+    """2D plain case: D2Q9 BGK (tau = 0.8) + D2Q5 fluid/solid temperature, gray disks
+    (radius 4-8 lu, solid fraction 1 - 0.65 inside), inlet u = 0.02 lu and T = 1,
+    outflow, walls T = 0."""
+    from .solver import LBM2DSolver
+    dx = dt = 1e-4
+    lb = LBM2DSolver(nx * dx, ny * dx, 1, dx=dx, dt=dt, name="plain2d", isThermal=True, isChemical=False,
+                     isPoro=True, isRadiation=False, precision=precision, lib=lib)
+    lb.source_term_model = SOURCE_TERM.MICRO
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(1e-5)
+    solid, mask = gray_packing((lb.nx, lb.ny, 1), 4.0, 8.0, 0.5, seed, 1 - 0.65)
+    lb.add_solid(solid)
+    lb.add_rho_solid(mask * 1.0)
+    lb.set_heat_exchange_surface(mask * 1.0)
+    lb.set_heat_exchange_coef(mask * 5e4)
+    lb.set_fluid_Trange([300.0, 800.0])
+    lb.set_solid_Trange([300.0, 800.0])
+    lb.set_fluid_Prandtl(0.71)
+    lb.set_fluid_capacity(1000.0)
+    lb.set_solid_thermal_diff(2e-6)
+    lb.set_solid_capacity(1500.0)
+    lb.set_BC(0, BC_FLOW.inlet)
+    lb.set_v_BC_value(0, [0.02, 0.0, 0.0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    lb.set_BC(2, BC_FLOW.wall)
+    lb.set_BC(3, BC_FLOW.wall)
+    lb.set_TF_BC(0, BC.fixedValue)
+    lb.set_TF_BC_value(0, 800.0)
+    lb.set_TF_BC(1, BC.zeroGradient)
+    lb.set_TF_BC(2, BC.fixedValue)
+    lb.set_TF_BC_value(2, 300.0)
+    lb.set_TF_BC(3, BC.fixedValue)
+    lb.set_TF_BC_value(3, 300.0)
+    lb.init_field(lb.rho, 1.0)
+    lb.init_field(lb.TF.S, 0.0)
+    lb.init_field(lb.TS.S, 0.0)
+    return lb
+
+
+def pyrolysis_kinetics(seed, n=3, T_hot=800.0, dt=1e-4, max_k_dt=0.01):
+    """Seeded (Ea, A) pairs: Ea ~ U(80, 180) kJ/mol, log10 A ~ U(5, 13), redrawn until
+    the explicit per-step conversion at T_hot stays below max_k_dt (the reference
+    integrates the source explicitly, LBM3D/_chemical.py:226)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        Ea = rng.uniform(80e3, 180e3)
+        A = 10 ** rng.uniform(5, 13)
+        k = A * np.exp(-Ea / (T_hot * R_GAS))
+        if k * dt <= max_k_dt and k * dt >= max_k_dt * 1e-4:
+            out.append((A, Ea))
+    return out
+
+
+SPECIES_3D = (("volatiles", False, 30.0), ("tar", False, 110.0), ("gas", False, 24.0), ("inert", False, 28.0),
+              ("biomass", True, 162.0), ("char", True, 12.0))
+
+
+def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", collision="bgk"):
+    """3D coupled case: D3Q19 flow, D3Q7 fluid/solid temperature and 4 species
+    (volatiles, tar, gas, inert) plus FIX biomass/char; gray spheres (radius 6-12 lu,
+    solid fraction 0.5 inside) with biomass density 1.0; three competing pyrolysis
+    reactions biomass -> tar, biomass -> gas, biomass -> 0.35 char + 0.65 volatiles;
+    heated inlet T = 1 (800 K)."""
+    from .solver import LBM3DSolver
+    dx = dt = 1e-4
+    lb = LBM3DSolver(nx * dx, ny * dx, nz * dx, dx=dx, dt=dt, name="coupled3d", isThermal=True, isChemical=True,
+                     isPoro=True, isRadiation=False, precision=precision, collision=collision, lib=lib)
+    # int(X/dx) may truncate a float product; pin the requested shape
+    assert (lb.nx, lb.ny, lb.nz) == (nx, ny, nz), (lb.nx, lb.ny, lb.nz)
+    lb.source_term_model = SOURCE_TERM.MICRO
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(1e-5)
+    solid, mask = gray_packing((nx, ny, nz), 6.0, 12.0, 0.5, seed, 0.5)
+    lb.add_solid(solid)
+    lb.set_poro_Darcy(mask * 1.0, 2e6)
+    lb.set_heat_exchange_surface(mask * 1.0)
+    lb.set_heat_exchange_coef(mask * 5e4)
+    lb.set_fluid_Trange([300.0, 800.0])
+    lb.set_solid_Trange([300.0, 800.0])
+    lb.set_fluid_Prandtl(0.71)
+    lb.set_fluid_capacity(1100.0)
+    lb.set_solid_thermal_diff(2e-6)
+    lb.set_solid_capacity(1500.0)
+    for name, fix, mm in SPECIES_3D:
+        lb.set_specie(name, fix, mm)
+        if not fix:
+            lb.set_specie_diff(name, 2e-5)
+        lb.set_specie_enthalpy(name, 0.0 if name != "biomass" else -1.0e4)
+        lb.set_specie_capacity(name, 30.0)
+    kin = pyrolysis_kinetics(seed)
+    lb.add_reaction("biomass => tar", kin[0][0], kin[0][1])
+    lb.add_reaction("biomass => gas", kin[1][0], kin[1][1])
+    lb.add_reaction("biomass => 0.35 char + 0.65 volatiles", kin[2][0], kin[2][1])
+    lb.init_specie("inert", 1.0)
+    lb.init_specie("biomass", mask * 1.0)
+    lb.set_BC(0, BC_FLOW.inlet)
+    lb.set_v_BC_value(0, [0.02, 0.0, 0.0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    for f in range(2, 6):
+        lb.set_BC(f, BC_FLOW.wall)
+    lb.set_TF_BC(0, BC.fixedValue)
+    lb.set_TF_BC_value(0, 800.0)
+    lb.set_TS_BC(0, BC.fixedValue)
+    lb.set_TS_BC_value(0, 800.0)
+    lb.set_TF_BC(1, BC.zeroGradient)
+    lb.set_species_BC(1, BC.zeroGradient)
+    for name, fix, _ in SPECIES_3D:
+        if not fix:
+            lb.set_specie_BC(name, 0, BC.fixedValue)
+            lb.set_specie_BC_value(name, 0, 1.0 if name == "inert" else 0.0)
+    lb.init_field(lb.rho, 1.0)
+    return lb
+
+
+# algorithmic HBM bytes per cell-step (populations read once and written once,
+# plus irreducible per-cell auxiliary reads / read-writes)
+def bytes_per_cell(q_tot, pop_bytes, n_ro, n_rw, aux_bytes):
+    return 2 * pop_bytes * q_tot + aux_bytes * (n_ro + 2 * n_rw)
+
+
+COUPLED3D_Q = 19 + 7 + 7 + 4 * 7  # 61
+COUPLED3D_BYTES_FP32 = bytes_per_cell(COUPLED3D_Q, 4, 3, 3, 4)  # 524
+PLAIN2D_BYTES_FP64 = bytes_per_cell(9 + 5 + 5, 8, 3, 0, 8)  # 328

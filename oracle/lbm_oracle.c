@@ -1,0 +1,1141 @@
+/*
+ * lbm_oracle.c -- CPU restatement of the reference's coupled LBM step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests compare
+ * the HIP library against, and the "port" CPU baseline that bench.py times.
+ * It is never linked into, loaded by, or used as a fallback of the product
+ * library (lbm_porouschem_amd/).
+ *
+ * It restates YueZeshi/LBM_porousChem (v0.1.1) literally, in the reference's own
+ * layout and order: two lattices (f/F, g/G), AoS populations (Q contiguous per
+ * cell), push streaming with periodic wrap and half-way bounce-back, then macro,
+ * then the non-equilibrium-extrapolation (NEE) face loops x -> y -> z.  Every
+ * per-cell helper is evaluated where the reference evaluates it (tau(i) per
+ * direction, force(i) per direction), so the arithmetic order follows the
+ * reference expression by expression.
+ *
+ * Reference files followed (paths under src/LBM/):
+ *   LBM3D/_evolution.py   step / collision_source_streaming / macro / feq19 /
+ *                         viscosity / tau / perm / darcy_force / forceTermGuo
+ *   LBM2D/_evolution.py   2D twin (D2Q9 + D2Q5)
+ *   LBM3D/_boundary.py, LBM2D/_boundary.py      NEE faces, 2D inflow controller
+ *   LBM3D/_scalarField.py, LBM2D/_scalarField.py scalar tau and scalar NEE
+ *   LBM3D/_thermal.py, LBM2D/_thermal.py         TF/TS equilibria and models
+ *   LBM3D/_chemical.py, LBM2D/_chemical.py       species, Arrhenius reactions
+ *   LBM3D/_initialize.py, LBM2D/_initialize.py   lattice constants, init
+ *
+ * Deliberate deviations (both are compile-blocking bugs in the reference):
+ *   D1  the solid-reactant rate uses species[j].molemass
+ *       (reference: self.LBM.species.molemass, LBM3D/_chemical.py:215)
+ *   D2  the 3D reaction source vector has n_species+1 entries as in 2D
+ *       (reference: LBM3D/_initialize.py:22 allocates n_species)
+ *
+ * Build: REAL defaults to double; -DORACLE_REAL=float builds the FP32 variant
+ * used as the CPU baseline of the FP32 workload.  The recipe is oracle/Makefile.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+
+#include "../include/lbm_porouschem.h"
+
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+typedef ORACLE_REAL real;
+
+/* util/constant.py:17,19 */
+#define GAS_R 8.314441
+#define SIGMA_SB 5.670374419e-8
+
+typedef struct scalar_lattice {
+    real* S;
+    real* dS;
+    real* g; /* [N][Qs] (NULL for FIX species) */
+    real* G;
+} scalar_lattice;
+
+struct lbm_handle {
+    lbm_desc d;
+    lbm_params p;
+    int params_set, initialised;
+    char err[512];
+    long long steps;
+    int nx, ny, nz;
+    size_t N;
+    int Q, Qs;
+    int e[19][3];
+    real w[19];
+    int LR[19];
+    int es[7][3];
+    real ws[7];
+    /* flow */
+    real *f, *F, *rho, *v, *solid, *rhos, *cd, *cf;
+    /* thermal */
+    scalar_lattice TF, TS;
+    real *exch_coef, *exch_surf, *rad_surf, *emis;
+    /* species */
+    scalar_lattice sp[LBM_MAX_SPECIES];
+    /* face profiles */
+    real* vprof[6];
+    real* rprof[6];
+    size_t face_n[6];
+};
+
+static int set_err(lbm_handle* h, int code, const char* fmt, ...) {
+    if (h) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(h->err, sizeof(h->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+/* ------------------------------------------------------------------ */
+/* indexing                                                             */
+/* ------------------------------------------------------------------ */
+static inline size_t IDX(const lbm_handle* h, int x, int y, int z) {
+    return ((size_t)x * h->ny + y) * h->nz + z;
+}
+static inline void COORD(const lbm_handle* h, size_t i, int* x, int* y, int* z) {
+    *z = (int)(i % h->nz);
+    size_t r = i / h->nz;
+    *y = (int)(r % h->ny);
+    *x = (int)(r / h->ny);
+}
+
+/* LBM3D/_evolution.py:201-210 (2D :206-212 wraps x and y only; nz = 1 there) */
+static inline size_t periodic_neighbor(const lbm_handle* h, int x, int y, int z, const int* e) {
+    int a = x + e[0], b = y + e[1], c = z + e[2];
+    if (a < 0) a = h->nx - 1;
+    if (a > h->nx - 1) a = 0;
+    if (b < 0) b = h->ny - 1;
+    if (b > h->ny - 1) b = 0;
+    if (h->d.dim == 3) {
+        if (c < 0) c = h->nz - 1;
+        if (c > h->nz - 1) c = 0;
+    }
+    return IDX(h, a, b, c);
+}
+
+static inline real dot3(const int* e, const real* u) {
+    return (real)e[0] * u[0] + (real)e[1] * u[1] + (real)e[2] * u[2];
+}
+static inline real dotr(const real* a, const real* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static int is_fix(const lbm_handle* h, int j) { return h->d.species_fix[j] != 0; }
+
+/* ------------------------------------------------------------------ */
+/* lattice constants: LBM3D/_initialize.py:25-43, LBM2D/_initialize.py:24-35, LR tables LBM3D/_core.py:36, LBM2D/_core.py:35 */
+/* ------------------------------------------------------------------ */
+static void static_init(lbm_handle* h) {
+    if (h->d.dim == 3) {
+        static const int e19[19][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+                                       {0, 0, 1},  {0, 0, -1}, {1, 1, 0},  {-1, 1, 0}, {1, -1, 0},
+                                       {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
+                                       {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+        static const int lr[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+        h->Q = 19;
+        h->Qs = 7;
+        for (int k = 0; k < 19; ++k) {
+            for (int a = 0; a < 3; ++a) h->e[k][a] = e19[k][a];
+            h->LR[k] = lr[k];
+            h->w[k] = k == 0 ? (real)(1.0 / 3.0) : (k < 7 ? (real)(1.0 / 18.0) : (real)(1.0 / 36.0));
+        }
+        for (int k = 0; k < 7; ++k) {
+            for (int a = 0; a < 3; ++a) h->es[k][a] = e19[k][a];
+            h->ws[k] = k == 0 ? (real)(1.0 / 4.0) : (real)(1.0 / 8.0);
+        }
+    } else {
+        /* ordering 0,E,N,W,S,NE,NW,SW,SE */
+        static const int e9[9][3] = {{0, 0, 0}, {1, 0, 0},  {0, 1, 0},   {-1, 0, 0}, {0, -1, 0},
+                                     {1, 1, 0}, {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
+        static const int lr[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+        h->Q = 9;
+        h->Qs = 5;
+        for (int k = 0; k < 9; ++k) {
+            for (int a = 0; a < 3; ++a) h->e[k][a] = e9[k][a];
+            h->LR[k] = lr[k];
+            h->w[k] = k == 0 ? (real)(4.0 / 9.0) : (k < 5 ? (real)(1.0 / 9.0) : (real)(1.0 / 36.0));
+        }
+        for (int k = 0; k < 5; ++k) {
+            for (int a = 0; a < 3; ++a) h->es[k][a] = e9[k][a];
+            h->ws[k] = k == 0 ? (real)(1.0 / 3.0) : (real)(1.0 / 6.0);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* physical helpers                                                     */
+/* ------------------------------------------------------------------ */
+static inline real phys(const lbm_thermal_params* t, real v) { return v * (real)t->v_scale + (real)t->v_ref; }
+/* dt/dx**2 and value*dt/dx**2 of the reference's Python-side constants (evaluated in double on the host) */
+static inline real dtdx2(const lbm_handle* h) { return (real)(h->p.dt / (h->p.dx * h->p.dx)); }
+static inline real lu(const lbm_handle* h, double v) { return (real)(v * h->p.dt / (h->p.dx * h->p.dx)); }
+
+static inline real GetTF(const lbm_handle* h, size_t i) {
+    return h->d.thermal ? phys(&h->p.tf, h->TF.S[i]) : (real)273.15;
+}
+static inline real GetTS(const lbm_handle* h, size_t i) {
+    return h->d.thermal ? phys(&h->p.ts, h->TS.S[i]) : (real)273.15;
+}
+
+/* Specie.viscosity, LBM3D/_chemical.py:70-77 */
+static real specie_viscosity(const lbm_handle* h, int j, size_t i) {
+    const lbm_species_params* s = &h->p.species[j];
+    real visco = (real)1e-5;
+    if (s->visc_model == LBM_VISC_CONSTANT) {
+        visco = lu(h, s->visco);
+    } else if (s->visc_model == LBM_VISC_SUTHERLAND) {
+        real T = GetTF(h, i);
+        visco = (real)s->sutherland[0] * (real)pow(T, 1.5) / (T + (real)s->sutherland[1]);
+    }
+    return visco;
+}
+
+/* viscosity / kinetic_viscosity / tau, LBM3D/_evolution.py:227-248 */
+static real viscosity(const lbm_handle* h, size_t i) {
+    real visco = (real)0.1;
+    int m = h->p.visc_model;
+    if (m == LBM_VISC_CONSTANT) {
+        visco = lu(h, h->p.visco);
+    } else if (m == LBM_VISC_SUTHERLAND) {
+        real T = GetTF(h, i);
+        visco = (real)h->p.sutherland[0] * (real)pow(T, 1.5) / (T + (real)h->p.sutherland[1]) * dtdx2(h);
+    } else if (m == LBM_VISC_MIXTURE) {
+        if (h->d.chemical)
+            for (int j = 0; j < h->d.n_species; ++j)
+                if (!is_fix(h, j)) visco += h->sp[j].S[i] * specie_viscosity(h, j, i);
+    }
+    return visco;
+}
+static real kinetic_viscosity(const lbm_handle* h, size_t i) {
+    real nu = viscosity(h, i);
+    if (h->p.eos == LBM_EOS_IDEAL_GAS) nu = nu / h->rho[i];
+    return nu;
+}
+static real tau_flow(const lbm_handle* h, size_t i) { return 3 * kinetic_viscosity(h, i) + (real)0.5; }
+
+/* feq19 / feq9, LBM3D/_evolution.py:213-225 */
+static real feq(const lbm_handle* h, int s, size_t i) {
+    real rho = h->rho[i];
+    const real* u = &h->v[3 * i];
+    real eps = 1 - h->solid[i];
+    real eu = dot3(h->e[s], u);
+    real uv = dotr(u, u);
+    real out = (real)1.0;
+    if (h->p.eos == LBM_EOS_INCOMPRESSIBLE)
+        out = h->w[s] * (rho + (real)3.0 * eu + (real)4.5 * eu * eu / (eps + (real)1e-12) -
+                         (real)1.5 * uv / (eps + (real)1e-12));
+    if (h->p.eos == LBM_EOS_IDEAL_GAS)
+        out = h->w[s] * rho *
+              ((real)1.0 + (real)3.0 * eu + (real)4.5 * eu * eu / (eps + (real)1e-12) -
+               (real)1.5 * uv / (eps + (real)1e-12));
+    return out;
+}
+
+/* perm, LBM3D/_evolution.py:250-252 */
+static inline real perm(real eps) {
+    real a = (real)1.000001 - eps;
+    return (real)pow(eps, 3) / (a * a) * 1;
+}
+
+/* darcy_force, LBM3D/_evolution.py:263-278 (3D: dynamic viscosity), LBM2D/_evolution.py:264-283 (kinematic) */
+static void darcy_force(const lbm_handle* h, size_t i, real* F) {
+    F[0] = F[1] = F[2] = (real)0.0;
+    if (!h->d.poro) return;
+    real sol = h->solid[i];
+    if (!(sol != 0 && sol != 1)) return;
+    const real* u = &h->v[3 * i];
+    real nu = h->d.dim == 3 ? viscosity(h, i) : kinetic_viscosity(h, i);
+    real eps = (real)1.0 - sol;
+    real c = 0;
+    switch (h->p.poro_model) {
+        case LBM_PORO_SPHERICAL: {
+            real pm = perm(eps);
+            real beta = (real)1.75 / (real)sqrt((real)150 * eps * pm);
+            real len = (real)sqrt(dotr(u, u));
+            c = -nu / pm * eps - beta * len;
+            break;
+        }
+        case LBM_PORO_DARCY:
+        case LBM_PORO_DARCY_HIGH:
+            if (h->p.poro_model == LBM_PORO_DARCY_HIGH && h->d.dim == 3) return;
+            c = -eps * nu * h->cd[i];
+            break;
+        case LBM_PORO_DARCYFORCHHEIMER: {
+            real len = (real)sqrt(dotr(u, u));
+            c = -eps * nu * h->cd[i] - eps * h->cf[i] * len;
+            break;
+        }
+        default:
+            return;
+    }
+    F[0] = c * u[0];
+    F[1] = c * u[1];
+    F[2] = c * u[2];
+}
+
+/* forceTermGuo, LBM3D/_evolution.py:280-293; 2D :285-298 (INCOMPRESSIBLE without the 1e-12 guard) */
+static real force_guo(const lbm_handle* h, int s, size_t i) {
+    real rho = h->rho[i];
+    const real* u = &h->v[3 * i];
+    real F[3];
+    darcy_force(h, i, F); /* force(i) = volume_force (zero) + darcy_force */
+    real tau = tau_flow(h, i);
+    real eps = (real)1.0 - h->solid[i];
+    real ee = (h->p.eos == LBM_EOS_INCOMPRESSIBLE && h->d.dim == 2) ? eps : eps + (real)1e-12;
+    real em[3];
+    for (int a = 0; a < 3; ++a) em[a] = (real)h->e[s][a] - u[a] / ee;
+    real eu = dot3(h->e[s], u), eF = dot3(h->e[s], F);
+    real term = (real)0.0;
+    if (h->p.eos == LBM_EOS_INCOMPRESSIBLE)
+        term = ((real)1.0 - (real)1.0 / (real)2.0 / tau) * h->w[s] * ((real)3.0 * dotr(em, F) + (real)9.0 * eu * eF / ee);
+    else
+        term = ((real)1.0 - (real)1.0 / (real)2.0 / tau) * rho * h->w[s] *
+               ((real)3.0 * dotr(em, F) + (real)9.0 * eu * eF / ee);
+    return term;
+}
+
+/* ---- scalar equilibria ---- */
+/* TF.geq7 LBM3D/_thermal.py:54-60 ; TF.geq5 LBM2D/_thermal.py:55-61 */
+static real geq_tf(const lbm_handle* h, int k, real T, size_t i) {
+    const real* u = &h->v[3 * i];
+    real eu = dot3(h->es[k], u);
+    real uv = dotr(u, u);
+    if (h->d.dim == 3) return h->ws[k] * T * ((real)1.0 + (real)4.0 * eu + (real)8.0 * eu * eu - (real)2.0 * uv);
+    return h->ws[k] * T * ((real)1.0 + (real)3.0 * eu + (real)4.5 * eu * eu - (real)1.5 * uv);
+}
+/* TS.geq, LBM3D/_thermal.py:164-167 */
+static real geq_ts(const lbm_handle* h, int k, real T) { return h->ws[k] * T; }
+/* Specie.geq7 LBM3D/_chemical.py:131-137 (quadratic); Specie.geq5 LBM2D/_chemical.py:130-135 (linear) */
+static real geq_sp(const lbm_handle* h, int k, real S, size_t i) {
+    const real* u = &h->v[3 * i];
+    real eu = dot3(h->es[k], u);
+    if (h->d.dim == 3) {
+        real uv = dotr(u, u);
+        return h->ws[k] * S * (1 + (real)4.0 * eu + (real)8.0 * eu * eu - 2 * uv);
+    }
+    return h->ws[k] * S * (1 + 3 * eu);
+}
+
+/* ---- thermal property models ---- */
+/* TemperatureFluid.conductivity LBM3D/_thermal.py:72-84 ; Specie.conductivity LBM3D/_chemical.py:120-127 */
+static real specie_conductivity(const lbm_handle* h, int j, size_t i) {
+    const lbm_species_params* s = &h->p.species[j];
+    real cond = (real)0.0;
+    if (s->cond_model == LBM_COND_CONSTANT) {
+        cond += (real)s->cond;
+    } else if (s->cond_model == LBM_COND_POLYNOMIAL) {
+        real T = GetTF(h, i);
+        const double* c = s->cond_poly;
+        cond += ((((real)c[4] * T + (real)c[3]) * T + (real)c[2]) * T + (real)c[1]) * T + (real)c[0];
+    }
+    return cond;
+}
+/* Specie.capacity_m(T) = capacity_mole/molemass; only CONSTANT is reachable (D3) */
+static real specie_capacity_m(const lbm_handle* h, int j) {
+    return (real)h->p.species[j].capa / (real)h->p.species[j].molemass;
+}
+
+static real poly5_ref(const double* c, real T) {
+    /* c0+T*(c1+T*(c2*T+T*(c3+T*c4))) exactly as written in LBM3D/_thermal.py:78,92 */
+    return (real)c[0] + T * ((real)c[1] + T * ((real)c[2] * T + T * ((real)c[3] + T * (real)c[4])));
+}
+
+static real tf_conductivity(const lbm_handle* h, size_t i) {
+    const lbm_thermal_params* t = &h->p.tf;
+    real kappa = (real)0.0;
+    if (t->cond_model == LBM_COND_CONSTANT) {
+        kappa = (real)t->cond;
+    } else if (t->cond_model == LBM_COND_POLYNOMIAL) {
+        kappa = poly5_ref(t->cond_poly, phys(t, h->TF.S[i]));
+    } else if (t->cond_model == LBM_COND_MIXTURE) {
+        if (h->d.chemical)
+            for (int j = 0; j < h->d.n_species; ++j)
+                if (!is_fix(h, j)) kappa += specie_conductivity(h, j, i) * h->sp[j].S[i];
+    }
+    return kappa;
+}
+/* TemperatureFluid.capacity_m LBM3D/_thermal.py:86-106 */
+static real tf_capacity(const lbm_handle* h, size_t i) {
+    const lbm_thermal_params* t = &h->p.tf;
+    real cm = (real)0.0;
+    if (t->cap_model == LBM_THERMO_CONSTANT) {
+        cm = (real)t->cm;
+    } else if (t->cap_model == LBM_THERMO_POLYNOMIAL) {
+        cm = poly5_ref(t->cm_poly, phys(t, h->TF.S[i]));
+    } else if (t->cap_model == LBM_THERMO_NASA7) {
+        real T = phys(t, h->TF.S[i]);
+        cm = T < (real)t->Trange[1] ? poly5_ref(t->nasa[0], T) : poly5_ref(t->nasa[1], T);
+        cm *= (real)GAS_R;
+    } else if (t->cap_model == LBM_THERMO_MIXTURE) {
+        if (h->d.chemical)
+            for (int j = 0; j < h->d.n_species; ++j)
+                if (!is_fix(h, j)) cm += specie_capacity_m(h, j) * h->sp[j].S[i];
+    }
+    return cm;
+}
+/* TemperatureFluid.coefDiff LBM3D/_thermal.py:62-70 */
+static real tf_coefdiff(const lbm_handle* h, size_t i) {
+    const lbm_thermal_params* t = &h->p.tf;
+    real D = (real)0.0;
+    if (t->diff_model == LBM_TDIFF_CONSTANT)
+        D = lu(h, t->diff);
+    else if (t->diff_model == LBM_TDIFF_PRANDTL)
+        D = kinetic_viscosity(h, i) / (real)t->Pr;
+    else if (t->diff_model == LBM_TDIFF_DERIVED)
+        D = tf_conductivity(h, i) / h->rho[i] / tf_capacity(h, i) * dtdx2(h);
+    return D;
+}
+/* TemperatureSolid.conductivity LBM3D/_thermal.py:177-189 (4-term polynomial) */
+static real ts_conductivity(const lbm_handle* h, size_t i) {
+    const lbm_thermal_params* t = &h->p.ts;
+    real kappa = (real)0.0;
+    if (t->cond_model == LBM_COND_CONSTANT) {
+        kappa = (real)t->cond;
+    } else if (t->cond_model == LBM_COND_POLYNOMIAL) {
+        real T = phys(t, h->TS.S[i]);
+        const double* c = t->cond_poly;
+        kappa = (real)c[0] + T * ((real)c[1] + T * ((real)c[2] * T + T * ((real)c[3])));
+    } else if (t->cond_model == LBM_COND_MIXTURE) {
+        if (h->d.chemical)
+            for (int j = 0; j < h->d.n_species; ++j)
+                if (is_fix(h, j)) kappa += specie_conductivity(h, j, i) * h->sp[j].S[i] / h->rhos[i];
+    }
+    return kappa;
+}
+/* TemperatureSolid.capacity_m LBM3D/_thermal.py:191-211 (NASA7 without *R; MIXTURE sums fluid species, D18) */
+static real ts_capacity(const lbm_handle* h, size_t i) {
+    const lbm_thermal_params* t = &h->p.ts;
+    real cm = (real)0.0;
+    if (t->cap_model == LBM_THERMO_CONSTANT) {
+        cm = (real)t->cm;
+    } else if (t->cap_model == LBM_THERMO_POLYNOMIAL) {
+        cm = poly5_ref(t->cm_poly, phys(t, h->TS.S[i]));
+    } else if (t->cap_model == LBM_THERMO_NASA7) {
+        real T = phys(t, h->TS.S[i]);
+        cm = T < (real)t->Trange[1] ? poly5_ref(t->nasa[0], T) : poly5_ref(t->nasa[1], T);
+    } else if (t->cap_model == LBM_THERMO_MIXTURE) {
+        if (h->d.chemical) {
+            for (int j = 0; j < h->d.n_species; ++j)
+                if (!is_fix(h, j)) cm += specie_capacity_m(h, j) * h->sp[j].S[i] * h->rho[i];
+        } else {
+            cm += 1000;
+        }
+    }
+    return cm;
+}
+/* TemperatureSolid.coefDiff LBM3D/_thermal.py:169-175 (no PRANDTL branch) */
+static real ts_coefdiff(const lbm_handle* h, size_t i) {
+    const lbm_thermal_params* t = &h->p.ts;
+    real D = (real)0.0;
+    if (t->diff_model == LBM_TDIFF_CONSTANT)
+        D = lu(h, t->diff);
+    else if (t->diff_model == LBM_TDIFF_DERIVED)
+        D = ts_conductivity(h, i) / h->rhos[i] / ts_capacity(h, i) * dtdx2(h);
+    return D;
+}
+/* Specie.coefDiff LBM3D/_chemical.py:62-68 */
+static real sp_coefdiff(const lbm_handle* h, int j, size_t i) {
+    const lbm_species_params* s = &h->p.species[j];
+    real diff = (real)0.0;
+    if (s->diff_model == LBM_DIFF_CONSTANT)
+        diff += lu(h, s->diff);
+    else if (s->diff_model == LBM_DIFF_SCHMIDT)
+        diff += (real)s->Sc * specie_viscosity(h, j, i) * dtdx2(h);
+    return diff;
+}
+/* ScalarField.tau: 4D+1/2 in 3D (LBM3D/_scalarField.py:56-57), 3D+1/2 in 2D (LBM2D/_scalarField.py:57-58) */
+static inline real scalar_tau(const lbm_handle* h, real D) {
+    return h->d.dim == 3 ? 4 * D + (real)0.5 : 3 * D + (real)0.5;
+}
+
+/* TemperatureSolid.radiation LBM3D/_thermal.py:215-221 */
+static real ts_radiation(const lbm_handle* h, size_t i) {
+    real q = (real)0.0;
+    if (h->p.radiation_model == LBM_RAD_SURFACE_UNIFORM) {
+        real Ta = (real)h->p.t_ambient;
+        real T = phys(&h->p.ts, h->TS.S[i]);
+        q += h->emis[i] * (real)SIGMA_SB * h->rad_surf[i] / (real)h->p.dx * ((real)pow(Ta, 4) - (real)pow(T, 4));
+    }
+    return q;
+}
+
+/* ---- reactions ---- */
+/* Reaction.Arrehnius LBM3D/_chemical.py:189-202 */
+static real arrhenius(const lbm_handle* h, const lbm_reaction_params* r, size_t i) {
+    real k = (real)0.0;
+    if (h->d.thermal) {
+        real T = h->solid[i] > 0 ? GetTS(h, i) : GetTF(h, i);
+        if (T > (real)r->Tmin)
+            k = (real)r->A * (real)pow(T + (real)1e-6, (real)r->b) *
+                (real)exp(-(real)r->Ea / (T + (real)1e-6) / (real)GAS_R);
+    } else {
+        k = (real)r->A;
+    }
+    return k;
+}
+/* Reaction.reaction LBM3D/_chemical.py:204-235 (D1 fixed) ; out has n_species+1 entries */
+static void reaction(const lbm_handle* h, const lbm_reaction_params* r, size_t i, real* out) {
+    int ns = h->d.n_species;
+    for (int j = 0; j <= ns; ++j) out[j] = (real)0.0;
+    real kr = arrhenius(h, r, i);
+    for (int j = 0; j < ns; ++j) {
+        if ((real)r->coef_reactant[j] > 0) {
+            if (h->sp[j].S[i] > (real)h->p.tol) {
+                if ((real)r->coef_rate[j] != 0) {
+                    real M = (real)h->p.species[j].molemass;
+                    if (!is_fix(h, j))
+                        kr *= (real)pow(h->sp[j].S[i] * h->rho[i] / M, (real)r->coef_rate[j]);
+                    else
+                        kr *= (real)pow(h->sp[j].S[i] / M, (real)r->coef_rate[j]);
+                }
+            } else {
+                kr = 0;
+            }
+        }
+    }
+    real dH = (real)0.0;
+    for (int j = 0; j < ns; ++j) {
+        real coef = -(real)r->coef_reactant[j] + (real)r->coef_product[j];
+        if (coef != (real)0.) {
+            real ds = kr * coef * (real)h->p.dt * (real)h->p.species[j].molemass;
+            out[j] = ds;
+            dH += coef * (real)h->p.species[j].enthalpy; /* enthalpy_mole: CONSTANT (D3) */
+        }
+    }
+    if (h->d.thermal) {
+        dH *= -kr * (real)h->p.dt;
+        out[ns] = dH;
+    }
+}
+/* Reactions.update_dS LBM3D/_chemical.py:259-270 */
+static void update_dS(lbm_handle* h, size_t i) {
+    int ns = h->d.n_species;
+    real acc[LBM_MAX_SPECIES + 1], tmp[LBM_MAX_SPECIES + 1];
+    for (int j = 0; j <= ns; ++j) acc[j] = 0;
+    for (int r = 0; r < h->p.n_reactions; ++r) {
+        reaction(h, &h->p.reactions[r], i, tmp);
+        for (int j = 0; j <= ns; ++j) acc[j] += tmp[j];
+    }
+    for (int j = 0; j < ns; ++j) h->sp[j].dS[i] = acc[j];
+    if (h->d.thermal) {
+        if (h->solid[i] > 0)
+            h->TS.dS[i] += acc[ns] / ts_capacity(h, i) / h->rhos[i];
+        else
+            h->TF.dS[i] += acc[ns] / tf_capacity(h, i) / h->rho[i];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* the step phases                                                      */
+/* ------------------------------------------------------------------ */
+/* collision_source_streaming, LBM3D/_evolution.py:27-116 (2D :27-116) */
+static void collision_source_streaming(lbm_handle* h) {
+    const int Q = h->Q, Qs = h->Qs, ns = h->d.n_species;
+    const int therm = h->d.thermal, chem = h->d.chemical;
+    long long N = (long long)h->N;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < N; ++ii) {
+        size_t i = (size_t)ii;
+        int x, y, z;
+        COORD(h, i, &x, &y, &z);
+        if (h->solid[i] < 1) {
+            for (int k = 0; k < Q; ++k) {
+                real* fi = &h->f[i * Q];
+                real f = fi[k] - 1 / tau_flow(h, i) * (fi[k] - feq(h, k, i));
+                fi[k] = f;
+                if (k < Qs) {
+                    if (therm) {
+                        real* g = &h->TF.g[i * Qs];
+                        g[k] = g[k] - 1 / scalar_tau(h, tf_coefdiff(h, i)) * (g[k] - geq_tf(h, k, h->TF.S[i], i));
+                        if (h->solid[i] > 0.0) {
+                            real* gs = &h->TS.g[i * Qs];
+                            gs[k] = gs[k] - 1 / scalar_tau(h, ts_coefdiff(h, i)) * (gs[k] - geq_ts(h, k, h->TS.S[i]));
+                        }
+                    }
+                    if (chem)
+                        for (int j = 0; j < ns; ++j)
+                            if (!is_fix(h, j)) {
+                                real* g = &h->sp[j].g[i * Qs];
+                                g[k] = g[k] - 1 / scalar_tau(h, sp_coefdiff(h, j, i)) * (g[k] - geq_sp(h, k, h->sp[j].S[i], i));
+                            }
+                }
+                if (h->p.force_term == LBM_FORCE_GUO) fi[k] += force_guo(h, k, i);
+                if (therm && k < Qs) {
+                    h->TF.g[i * Qs + k] += geq_tf(h, k, h->TF.dS[i], i);
+                    if (h->solid[i] > 0.0) h->TS.g[i * Qs + k] += geq_ts(h, k, h->TS.dS[i]);
+                }
+                if (chem && k < Qs) {
+                    for (int j = 0; j < ns; ++j) {
+                        if (!is_fix(h, j)) {
+                            /* 3D adds TF.geq7(dS) (same formula as the species equilibrium); 2D adds specie.geq5(dS) */
+                            real src = h->d.dim == 3 ? geq_tf(h, k, h->sp[j].dS[i], i) : geq_sp(h, k, h->sp[j].dS[i], i);
+                            h->sp[j].g[i * Qs + k] += src;
+                        } else if (k == 0) {
+                            h->sp[j].S[i] += h->sp[j].dS[i];
+                        }
+                    }
+                }
+                size_t ip = periodic_neighbor(h, x, y, z, h->e[k]);
+                if (h->solid[ip] < 1) {
+                    h->F[ip * Q + k] = fi[k];
+                    if (k < Qs) {
+                        if (therm) h->TF.G[ip * Qs + k] = h->TF.g[i * Qs + k];
+                        if (chem)
+                            for (int j = 0; j < ns; ++j)
+                                if (!is_fix(h, j)) h->sp[j].G[ip * Qs + k] = h->sp[j].g[i * Qs + k];
+                    }
+                } else {
+                    int kk = h->LR[k];
+                    h->F[i * Q + kk] = fi[k];
+                    if (k < Qs) {
+                        if (therm) h->TF.G[i * Qs + kk] = h->TF.g[i * Qs + k];
+                        if (chem)
+                            for (int j = 0; j < ns; ++j)
+                                if (!is_fix(h, j)) h->sp[j].G[i * Qs + kk] = h->sp[j].g[i * Qs + k];
+                    }
+                }
+                if (h->solid[i] > 0.0 && k < Qs && therm) {
+                    if (h->solid[ip] > 0)
+                        h->TS.G[ip * Qs + k] = h->TS.g[i * Qs + k];
+                    else
+                        h->TS.G[i * Qs + h->LR[k]] = h->TS.g[i * Qs + k];
+                }
+            }
+        } else {
+            /* pure solid: only the solid temperature collides and streams (no source, D22) */
+            for (int k = 0; k < Q; ++k) {
+                if (k < Qs && therm) {
+                    real* gs = &h->TS.g[i * Qs];
+                    gs[k] = gs[k] - 1 / scalar_tau(h, ts_coefdiff(h, i)) * (gs[k] - geq_ts(h, k, h->TS.S[i]));
+                }
+                size_t ip = periodic_neighbor(h, x, y, z, h->e[k]);
+                if (k < Qs && therm) {
+                    if (h->solid[ip] > 0)
+                        h->TS.G[ip * Qs + k] = h->TS.g[i * Qs + k];
+                    else
+                        h->TS.G[i * Qs + h->LR[k]] = h->TS.g[i * Qs + k];
+                }
+            }
+        }
+    }
+}
+
+/* macro, LBM3D/_evolution.py:118-196 (2D :119-203) */
+static void macro(lbm_handle* h) {
+    const int Q = h->Q, Qs = h->Qs, ns = h->d.n_species;
+    long long N = (long long)h->N;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < N; ++ii) {
+        size_t i = (size_t)ii;
+        if (!(h->solid[i] < 1)) continue;
+        real eps = 1 - h->solid[i];
+        real* u = &h->v[3 * i];
+        u[0] = u[1] = u[2] = 0;
+        h->rho[i] = (real)0.0;
+        for (int s = 0; s < Q; ++s) {
+            real Fs = h->F[i * Q + s];
+            h->f[i * Q + s] = Fs;
+            u[0] += (real)h->e[s][0] * Fs;
+            u[1] += (real)h->e[s][1] * Fs;
+            u[2] += (real)h->e[s][2] * Fs;
+            h->rho[i] += Fs;
+        }
+        if (h->p.eos == LBM_EOS_IDEAL_GAS) {
+            u[0] /= h->rho[i];
+            u[1] /= h->rho[i];
+            u[2] /= h->rho[i];
+        }
+        if (h->d.poro && eps != 0 && eps != 1) {
+            real Dc = (real)0.0, Fc = (real)0.0;
+            if (h->p.poro_model == LBM_PORO_SPHERICAL) {
+                Dc = (real)1.0 / perm(eps);
+                Fc = (real)1.75 / (real)sqrt(150 * (real)pow(eps, 3));
+                real c0 = (real)0.5 * (1 + eps * kinetic_viscosity(h, i) / 2 * Dc);
+                real c1 = (real)0.5 * eps * Fc;
+                real vn = (real)sqrt(dotr(u, u));
+                real c = c0 + (real)sqrt(c0 * c0 + c1 * vn);
+                u[0] /= c; u[1] /= c; u[2] /= c;
+            } else if (h->p.poro_model == LBM_PORO_DARCY) {
+                Dc = h->cd[i];
+                real c = 1 + eps * kinetic_viscosity(h, i) * Dc / 2;
+                u[0] /= c; u[1] /= c; u[2] /= c;
+            } else if (h->p.poro_model == LBM_PORO_DARCYFORCHHEIMER) {
+                Dc = h->cd[i];
+                Fc = h->cf[i];
+                real c0 = (real)0.5 * (1 + eps * kinetic_viscosity(h, i) / 2 * Dc);
+                real c1 = (real)0.5 * eps * Fc;
+                real vn = (real)sqrt(dotr(u, u));
+                real c = c0 + (real)sqrt(c0 * c0 + c1 * vn);
+                u[0] /= c; u[1] /= c; u[2] /= c;
+            }
+        }
+        if (h->d.thermal) {
+            h->TS.dS[i] = (real)0.0;
+            h->TF.dS[i] = (real)0.0;
+        }
+        if (h->d.chemical) {
+            if (h->d.poro) h->rhos[i] = (real)0.0;
+            real Yall = (real)0.0;
+            for (int j = 0; j < ns; ++j) {
+                if (!is_fix(h, j)) {
+                    h->sp[j].S[i] = (real)0.0;
+                    for (int s = 0; s < Qs; ++s) {
+                        h->sp[j].g[i * Qs + s] = h->sp[j].G[i * Qs + s];
+                        h->sp[j].S[i] += h->sp[j].G[i * Qs + s];
+                    }
+                    if (h->sp[j].S[i] < -(real)h->p.tol) h->sp[j].S[i] = -(real)h->p.tol;
+                    Yall += h->sp[j].S[i];
+                } else if (h->d.poro) {
+                    h->rhos[i] += h->sp[j].S[i];
+                }
+            }
+            for (int j = 0; j < ns; ++j)
+                if (!is_fix(h, j)) h->sp[j].S[i] /= Yall;
+            update_dS(h, i);
+        }
+        if (h->d.thermal) {
+            real Tf = (real)0.0, Ts = (real)0.0;
+            for (int s = 0; s < Qs; ++s) {
+                h->TF.g[i * Qs + s] = h->TF.G[i * Qs + s];
+                h->TS.g[i * Qs + s] = h->TS.G[i * Qs + s];
+                Tf += h->TF.G[i * Qs + s];
+                Ts += h->TS.G[i * Qs + s];
+            }
+            h->TF.S[i] = Tf;
+            h->TS.S[i] = Ts;
+            if (h->solid[i] > 0) {
+                real dH = h->exch_coef[i] * h->exch_surf[i] * (phys(&h->p.tf, Tf) - phys(&h->p.ts, Ts)) * (real)h->p.dt;
+                h->TS.dS[i] += dH / ts_capacity(h, i) / h->rhos[i] / (real)h->p.ts.v_scale;
+                h->TF.dS[i] += -dH / tf_capacity(h, i) / h->rho[i] / (real)h->p.tf.v_scale;
+            }
+            if (h->d.radiation)
+                h->TS.dS[i] += ts_radiation(h, i) * (real)h->p.dt / ts_capacity(h, i) / h->rhos[i] / (real)h->p.ts.v_scale;
+        }
+    }
+}
+
+/* face bookkeeping: face 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+ */
+static size_t face_cell(const lbm_handle* h, int face, int x, int y, int z) {
+    if (face < 2) return (size_t)y * h->nz + z;
+    if (face < 4) return (size_t)x * h->nz + z;
+    return (size_t)x * h->ny + y;
+}
+
+/* Boundary_condition_flow_NEE_k, LBM3D/_boundary.py:211-282 ; LBM2D/_boundary.py:53-101 */
+static void nee_flow(lbm_handle* h, int face, size_t b, size_t nb, size_t fc) {
+    const int Q = h->Q;
+    int bv = h->p.bc_v[face], br = h->p.bc_rho[face];
+    if (bv == LBM_BC_FIXEDVALUE) {
+        for (int a = 0; a < 3; ++a) h->v[3 * b + a] = h->vprof[face][3 * fc + a];
+    } else if (bv == LBM_BC_ZEROGRADIENT) {
+        for (int a = 0; a < 3; ++a) h->v[3 * b + a] = h->v[3 * nb + a];
+    }
+    if (br == LBM_BC_FIXEDVALUE) {
+        /* 2D face 0 reads rho_bc_profile[0] (LBM2D/_boundary.py:58) */
+        h->rho[b] = (h->d.dim == 2 && face == 0) ? h->rprof[0][fc] : (real)h->p.rho_bc[face];
+    } else if (br == LBM_BC_ZEROGRADIENT) {
+        /* 3D z- is a no-op (D10, LBM3D/_boundary.py:268) */
+        if (!(h->d.dim == 3 && face == 4)) h->rho[b] = h->rho[nb];
+    }
+    for (int s = 0; s < Q; ++s)
+        h->f[b * Q + s] = feq(h, s, b) + (h->f[nb * Q + s] - feq(h, s, nb));
+}
+
+/* Boundary_condition_scalar_k, LBM3D/_scalarField.py:83-141 ; kind 0 = species j, 1 = TF, 2 = TS */
+static void nee_scalar(lbm_handle* h, int kind, int j, int face, size_t b, size_t nb) {
+    const int Qs = h->Qs;
+    scalar_lattice* L = kind == 0 ? &h->sp[j] : (kind == 1 ? &h->TF : &h->TS);
+    int type;
+    double value;
+    if (kind == 0) {
+        type = h->p.species[j].bc_type[face];
+        value = h->p.species[j].bc_value[face];
+    } else {
+        const lbm_thermal_params* t = kind == 1 ? &h->p.tf : &h->p.ts;
+        type = t->bc_type[face];
+        value = t->bc_value[face];
+    }
+    if (type == LBM_BC_FIXEDVALUE) L->S[b] = (real)value;
+    if (type == LBM_BC_ZEROGRADIENT) L->S[b] = L->S[nb];
+    for (int s = 0; s < Qs; ++s) {
+        real gb, gn;
+        if (kind == 0) {
+            gb = geq_sp(h, s, L->S[b], b);
+            gn = geq_sp(h, s, L->S[nb], nb);
+        } else if (kind == 1) {
+            gb = geq_tf(h, s, L->S[b], b);
+            gn = geq_tf(h, s, L->S[nb], nb);
+        } else {
+            gb = geq_ts(h, s, L->S[b]);
+            gn = geq_ts(h, s, L->S[nb]);
+        }
+        L->G[b * Qs + s] = gb + (L->G[nb * Qs + s] - gn);
+        L->g[b * Qs + s] = L->G[b * Qs + s];
+    }
+}
+
+static void nee_cell(lbm_handle* h, int face, int x, int y, int z) {
+    int nxp = x, nyp = y, nzp = z;
+    switch (face) {
+        case 0: nxp = 1; break;
+        case 1: nxp = h->nx - 2; break;
+        case 2: nyp = 1; break;
+        case 3: nyp = h->ny - 2; break;
+        case 4: nzp = 1; break;
+        case 5: nzp = h->nz - 2; break;
+    }
+    size_t b = IDX(h, x, y, z), nb = IDX(h, nxp, nyp, nzp);
+    nee_flow(h, face, b, nb, face_cell(h, face, x, y, z));
+    if (h->d.chemical)
+        for (int j = 0; j < h->d.n_species; ++j)
+            if (!is_fix(h, j)) nee_scalar(h, 0, j, face, b, nb);
+    if (h->d.thermal) {
+        nee_scalar(h, 1, 0, face, b, nb);
+        nee_scalar(h, 2, 0, face, b, nb);
+    }
+}
+
+/* Boundary_condition_NEE, LBM3D/_boundary.py:155-208 ; LBM2D/_boundary.py:16-51 */
+static void boundary_nee(lbm_handle* h) {
+    for (int y = 0; y < h->ny; ++y)
+        for (int z = 0; z < h->nz; ++z) {
+            nee_cell(h, 0, 0, y, z);
+            nee_cell(h, 1, h->nx - 1, y, z);
+        }
+    for (int x = 0; x < h->nx; ++x)
+        for (int z = 0; z < h->nz; ++z) {
+            nee_cell(h, 2, x, 0, z);
+            nee_cell(h, 3, x, h->ny - 1, z);
+        }
+    if (h->d.dim == 3)
+        for (int x = 0; x < h->nx; ++x)
+            for (int y = 0; y < h->ny; ++y) {
+                nee_cell(h, 4, x, y, 0);
+                nee_cell(h, 5, x, y, h->nz - 1);
+            }
+}
+
+/* 2D inflow-rate controller, LBM2D/_boundary.py:324-333 */
+static void update_bc_2d(lbm_handle* h) {
+    if (h->d.dim != 2 || h->p.bc_flow[0] != LBM_BCFLOW_INLET_FLOW) return;
+    real flow0 = (real)0.0;
+    for (int y = 0; y < h->ny; ++y) {
+        size_t b = IDX(h, 0, y, 0);
+        flow0 += h->rho[b] * h->v[3 * b];
+    }
+    real coef = (real)h->p.flow_bc[0] / flow0;
+    real a = (real).01;
+    for (int y = 0; y < h->ny; ++y) {
+        size_t n1 = IDX(h, 1, y, 0);
+        for (int c = 0; c < 3; ++c) h->vprof[0][3 * y + c] = (a * coef + 1 - a) * h->v[3 * n1 + c];
+        h->rprof[0][y] = h->rho[n1];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* C ABI                                                                */
+/* ------------------------------------------------------------------ */
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+static real* zalloc(size_t n) { return (real*)calloc(n ? n : 1, sizeof(real)); }
+
+int lbm_create(const lbm_desc* desc, lbm_handle** out) {
+    if (!desc || !out) return LBM_ERR_INVALID;
+    *out = NULL;
+    if (desc->abi_version != LBM_ABI_VERSION) return LBM_ERR_INVALID;
+    if (desc->dim != 2 && desc->dim != 3) return LBM_ERR_INVALID;
+    if (desc->nx < 4 || desc->ny < 4 || (desc->dim == 3 && desc->nz < 4) || (desc->dim == 2 && desc->nz != 1))
+        return LBM_ERR_INVALID;
+    if (desc->n_species < 0 || desc->n_species > LBM_MAX_SPECIES) return LBM_ERR_INVALID;
+    lbm_handle* h = (lbm_handle*)calloc(1, sizeof(lbm_handle));
+    if (!h) return LBM_ERR_OOM;
+    h->d = *desc;
+    h->nx = desc->nx;
+    h->ny = desc->ny;
+    h->nz = desc->nz;
+    h->N = (size_t)h->nx * h->ny * h->nz;
+    static_init(h);
+    size_t N = h->N;
+    int Q = h->Q, Qs = h->Qs;
+    h->f = zalloc(N * Q);
+    h->F = zalloc(N * Q);
+    h->rho = zalloc(N);
+    h->v = zalloc(3 * N);
+    h->solid = zalloc(N);
+    h->rhos = zalloc(N);
+    h->cd = zalloc(N);
+    h->cf = zalloc(N);
+    h->TF.S = zalloc(N); h->TF.dS = zalloc(N); h->TF.g = zalloc(N * Qs); h->TF.G = zalloc(N * Qs);
+    h->TS.S = zalloc(N); h->TS.dS = zalloc(N); h->TS.g = zalloc(N * Qs); h->TS.G = zalloc(N * Qs);
+    h->exch_coef = zalloc(N); h->exch_surf = zalloc(N); h->rad_surf = zalloc(N); h->emis = zalloc(N);
+    int ok = h->f && h->F && h->rho && h->v && h->solid && h->rhos && h->cd && h->cf && h->TF.S && h->TF.dS &&
+             h->TF.g && h->TF.G && h->TS.S && h->TS.dS && h->TS.g && h->TS.G && h->exch_coef && h->exch_surf &&
+             h->rad_surf && h->emis;
+    for (int j = 0; j < desc->n_species && ok; ++j) {
+        h->sp[j].S = zalloc(N);
+        h->sp[j].dS = zalloc(N);
+        ok = h->sp[j].S && h->sp[j].dS;
+        if (!desc->species_fix[j]) {
+            h->sp[j].g = zalloc(N * Qs);
+            h->sp[j].G = zalloc(N * Qs);
+            ok = ok && h->sp[j].g && h->sp[j].G;
+        }
+    }
+    for (int fce = 0; fce < 2 * desc->dim && ok; ++fce) {
+        size_t n = fce < 2 ? (size_t)h->ny * h->nz : (fce < 4 ? (size_t)h->nx * h->nz : (size_t)h->nx * h->ny);
+        h->face_n[fce] = n;
+        h->vprof[fce] = zalloc(3 * n);
+        h->rprof[fce] = zalloc(n);
+        ok = h->vprof[fce] && h->rprof[fce];
+    }
+    if (!ok) {
+        lbm_destroy(h);
+        return LBM_ERR_OOM;
+    }
+    /* default_init: rho = 1 (LBM3D/_initialize.py:8-16) */
+    for (size_t i = 0; i < N; ++i) h->rho[i] = 1;
+    *out = h;
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_handle* h) {
+    if (!h) return;
+    free(h->f); free(h->F); free(h->rho); free(h->v); free(h->solid); free(h->rhos); free(h->cd); free(h->cf);
+    free(h->TF.S); free(h->TF.dS); free(h->TF.g); free(h->TF.G);
+    free(h->TS.S); free(h->TS.dS); free(h->TS.g); free(h->TS.G);
+    free(h->exch_coef); free(h->exch_surf); free(h->rad_surf); free(h->emis);
+    for (int j = 0; j < LBM_MAX_SPECIES; ++j) {
+        free(h->sp[j].S); free(h->sp[j].dS); free(h->sp[j].g); free(h->sp[j].G);
+    }
+    for (int fce = 0; fce < 6; ++fce) { free(h->vprof[fce]); free(h->rprof[fce]); }
+    free(h);
+}
+
+int lbm_set_params(lbm_handle* h, const lbm_params* p) {
+    if (!h || !p) return LBM_ERR_INVALID;
+    if (p->n_reactions < 0 || p->n_reactions > LBM_MAX_REACTIONS)
+        return set_err(h, LBM_ERR_INVALID, "n_reactions out of range");
+    if (h->d.collision != LBM_COLL_BGK)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "the oracle restates the reference BGK collision only");
+    h->p = *p;
+    h->params_set = 1;
+    return LBM_OK;
+}
+
+static real* cell_field(lbm_handle* h, int field, int comp, int* stride) {
+    *stride = 1;
+    switch (field) {
+        case LBM_FIELD_RHO: return h->rho;
+        case LBM_FIELD_VEL: if (comp < 0 || comp > 2) return NULL; *stride = 3; return h->v + comp;
+        case LBM_FIELD_SOLID: return h->solid;
+        case LBM_FIELD_RHOS: return h->rhos;
+        case LBM_FIELD_COEF_DARCY: return h->cd;
+        case LBM_FIELD_COEF_FORCH: return h->cf;
+        case LBM_FIELD_TF: return h->TF.S;
+        case LBM_FIELD_TS: return h->TS.S;
+        case LBM_FIELD_TF_DS: return h->TF.dS;
+        case LBM_FIELD_TS_DS: return h->TS.dS;
+        case LBM_FIELD_EXCH_SURF: return h->exch_surf;
+        case LBM_FIELD_EXCH_COEF: return h->exch_coef;
+        case LBM_FIELD_RAD_SURF: return h->rad_surf;
+        case LBM_FIELD_EMISSIVITY: return h->emis;
+        case LBM_FIELD_SPECIES: if (comp < 0 || comp >= h->d.n_species) return NULL; return h->sp[comp].S;
+        case LBM_FIELD_SPECIES_DS: if (comp < 0 || comp >= h->d.n_species) return NULL; return h->sp[comp].dS;
+        default: return NULL;
+    }
+}
+
+static int field_ptr(lbm_handle* h, int field, int comp, real** ptr, int* stride, size_t* count) {
+    if (field == LBM_FIELD_V_BC_PROFILE) {
+        int face = comp / 3, ax = comp % 3;
+        if (comp < 0 || face >= 2 * h->d.dim) return LBM_ERR_INVALID;
+        *ptr = h->vprof[face] + ax;
+        *stride = 3;
+        *count = h->face_n[face];
+        return LBM_OK;
+    }
+    if (field == LBM_FIELD_RHO_BC_PROFILE) {
+        if (comp < 0 || comp >= 2 * h->d.dim) return LBM_ERR_INVALID;
+        *ptr = h->rprof[comp];
+        *stride = 1;
+        *count = h->face_n[comp];
+        return LBM_OK;
+    }
+    *ptr = cell_field(h, field, comp, stride);
+    *count = h->N;
+    return *ptr ? LBM_OK : LBM_ERR_INVALID;
+}
+
+int lbm_set_field(lbm_handle* h, int field, int comp, const void* host, int dtype, size_t n) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    real* dst;
+    int stride;
+    size_t count;
+    if (field_ptr(h, field, comp, &dst, &stride, &count) != LBM_OK)
+        return set_err(h, LBM_ERR_INVALID, "unknown field %d comp %d", field, comp);
+    if (n != count) return set_err(h, LBM_ERR_INVALID, "field %d expects %zu values, got %zu", field, count, n);
+    for (size_t i = 0; i < n; ++i)
+        dst[i * stride] = dtype == LBM_DTYPE_F64 ? (real)((const double*)host)[i] : (real)((const float*)host)[i];
+    return LBM_OK;
+}
+
+int lbm_get_field(lbm_handle* h, int field, int comp, void* host, int dtype, size_t n) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    real* src;
+    int stride;
+    size_t count;
+    if (field_ptr(h, field, comp, &src, &stride, &count) != LBM_OK)
+        return set_err(h, LBM_ERR_INVALID, "unknown field %d comp %d", field, comp);
+    if (n != count) return set_err(h, LBM_ERR_INVALID, "field %d holds %zu values, asked %zu", field, count, n);
+    for (size_t i = 0; i < n; ++i) {
+        if (dtype == LBM_DTYPE_F64)
+            ((double*)host)[i] = (double)src[i * stride];
+        else
+            ((float*)host)[i] = (float)src[i * stride];
+    }
+    return LBM_OK;
+}
+
+/* init_kernel + init_boundary, LBM3D/_initialize.py:45-88 */
+int lbm_init(lbm_handle* h) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->params_set) return set_err(h, LBM_ERR_STATE, "lbm_set_params must precede lbm_init");
+    const int Q = h->Q, Qs = h->Qs, ns = h->d.n_species;
+    for (size_t i = 0; i < h->N; ++i) {
+        if (h->d.poro && h->d.chemical) {
+            h->rhos[i] = (real)0.0;
+            for (int j = 0; j < ns; ++j)
+                if (is_fix(h, j)) h->rhos[i] += h->sp[j].S[i];
+        }
+        for (int s = 0; s < Q; ++s) {
+            h->f[i * Q + s] = feq(h, s, i);
+            h->F[i * Q + s] = h->f[i * Q + s];
+        }
+        for (int s = 0; s < Qs; ++s) {
+            if (h->d.chemical)
+                for (int j = 0; j < ns; ++j)
+                    if (!is_fix(h, j)) {
+                        h->sp[j].g[i * Qs + s] = geq_sp(h, s, h->sp[j].S[i], i);
+                        h->sp[j].G[i * Qs + s] = h->sp[j].g[i * Qs + s];
+                    }
+            if (h->d.thermal) {
+                h->TF.g[i * Qs + s] = geq_tf(h, s, h->TF.S[i], i);
+                h->TF.G[i * Qs + s] = h->TF.g[i * Qs + s];
+                h->TS.g[i * Qs + s] = geq_ts(h, s, h->TS.S[i]);
+                h->TS.G[i * Qs + s] = h->TS.g[i * Qs + s];
+            }
+        }
+    }
+    if (h->p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW) {
+        real v0 = (real)h->p.flow_bc[0] / h->ny / h->nz;
+        for (int y = 0; y < h->ny; ++y)
+            for (int z = 0; z < h->nz; ++z) {
+                size_t fc = face_cell(h, 0, 0, y, z);
+                h->vprof[0][3 * fc] = v0; h->vprof[0][3 * fc + 1] = 0; h->vprof[0][3 * fc + 2] = 0;
+                size_t b0 = IDX(h, 0, y, z), b1 = IDX(h, 1, y, z);
+                h->v[3 * b0] = v0; h->v[3 * b0 + 1] = 0; h->v[3 * b0 + 2] = 0;
+                h->v[3 * b1] = v0; h->v[3 * b1 + 1] = 0; h->v[3 * b1 + 2] = 0;
+            }
+    }
+    if (h->p.bc_flow[0] == LBM_BCFLOW_INLET) {
+        for (size_t fc = 0; fc < h->face_n[0]; ++fc)
+            for (int a = 0; a < 3; ++a) h->vprof[0][3 * fc + a] = (real)h->p.v_bc[0][a];
+    }
+    h->initialised = 1;
+    h->steps = 0;
+    return LBM_OK;
+}
+
+static int check_div(lbm_handle* h) {
+    if (!h->p.check_divergence) return LBM_OK;
+    for (size_t i = 0; i < h->N; ++i) {
+        const real* u = &h->v[3 * i];
+        real m = (real)sqrt(dotr(u, u));
+        if (!(m <= (real)h->p.max_velocity))
+            return set_err(h, LBM_ERR_DIVERGED, "divergence at cell %zu after step %lld (|u| = %g)", i, h->steps,
+                           (double)m);
+    }
+    return LBM_OK;
+}
+
+int lbm_step(lbm_handle* h, int nsteps) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->initialised) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_step");
+    if (nsteps < 0) return set_err(h, LBM_ERR_INVALID, "nsteps < 0");
+    for (int n = 0; n < nsteps; ++n) {
+        update_bc_2d(h);
+        collision_source_streaming(h);
+        macro(h);
+        if (h->p.bc_model == LBM_BCMODEL_NEE) boundary_nee(h);
+        h->steps++;
+    }
+    return check_div(h);
+}
+
+int lbm_reduce(lbm_handle* h, int kind, int comp, double* out) {
+    if (!h || !out) return LBM_ERR_INVALID;
+    double acc;
+    switch (kind) {
+        case LBM_REDUCE_MAX_V:
+            acc = -1e10;
+            for (size_t i = 0; i < h->N; ++i) {
+                double m = sqrt((double)dotr(&h->v[3 * i], &h->v[3 * i]));
+                if (m > acc) acc = m;
+            }
+            break;
+        case LBM_REDUCE_MIN_TF:
+            acc = 1e10;
+            for (size_t i = 0; i < h->N; ++i) if ((double)h->TF.S[i] < acc) acc = h->TF.S[i];
+            break;
+        case LBM_REDUCE_MAX_TF:
+            acc = -1e10;
+            for (size_t i = 0; i < h->N; ++i) if ((double)h->TF.S[i] > acc) acc = h->TF.S[i];
+            break;
+        case LBM_REDUCE_MASS:
+            acc = 0;
+            for (size_t i = 0; i < h->N; ++i) if (h->solid[i] < 1) acc += h->rho[i];
+            break;
+        case LBM_REDUCE_TF_SUM:
+            acc = 0;
+            for (size_t i = 0; i < h->N; ++i) if (h->solid[i] < 1) acc += h->TF.S[i];
+            break;
+        case LBM_REDUCE_TS_SUM:
+            acc = 0;
+            for (size_t i = 0; i < h->N; ++i) if (h->solid[i] > 0) acc += h->TS.S[i];
+            break;
+        case LBM_REDUCE_FIX_SUM:
+            acc = 0;
+            for (int j = 0; j < h->d.n_species; ++j)
+                if (is_fix(h, j))
+                    for (size_t i = 0; i < h->N; ++i) acc += h->sp[j].S[i];
+            break;
+        case LBM_REDUCE_SPECIES_SUM:
+            if (comp < 0 || comp >= h->d.n_species) return set_err(h, LBM_ERR_INVALID, "species %d", comp);
+            acc = 0;
+            for (size_t i = 0; i < h->N; ++i) if (h->solid[i] < 1) acc += h->sp[comp].S[i];
+            break;
+        default:
+            return set_err(h, LBM_ERR_INVALID, "unknown reduction %d", kind);
+    }
+    *out = acc;
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_handle* h) { return h ? LBM_OK : LBM_ERR_INVALID; }
+long long lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+int lbm_profile(lbm_handle* h, int enable) { (void)enable; return h ? LBM_OK : LBM_ERR_INVALID; }
+int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* launches) {
+    (void)kernel;
+    if (!h || !total_ms || !launches) return LBM_ERR_INVALID;
+    *total_ms = 0;
+    *launches = 0;
+    return LBM_OK;
+}
+size_t lbm_struct_size(int which) { return which == 0 ? sizeof(lbm_desc) : (which == 1 ? sizeof(lbm_params) : 0); }
+const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null handle"; }

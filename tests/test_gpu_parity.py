@@ -1,0 +1,80 @@
+"""Parity of the HIP library against the CPU restatement of the reference step.
+
+FP64 kernels: relative L2 <= 1e-10 on every macroscopic field after each block of
+steps.  FP32 kernels: relative L2 <= 1e-5 against the FP64 checker on velocity,
+temperatures, species and solid density after 20 steps (and <= 2e-5 after 100)."""
+import numpy as np
+import pytest
+
+from cases import build, fields, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "2d_flow": dict(dim=2, n=(24, 16), therm=False, poro=False, solidmode="none"),
+    "2d_gray_darcy": dict(dim=2, n=(24, 16), therm=False, poro=True),
+    "2d_thermal_gray": dict(dim=2, n=(24, 16), therm=True, poro=True),
+    "2d_thermal_mixed_solid": dict(dim=2, n=(24, 16), therm=True, poro=True, solidmode="mixed"),
+    "2d_chem": dict(dim=2, n=(24, 16), therm=True, poro=True, chem=True),
+    "3d_flow": dict(dim=3, n=(12, 10, 9), therm=False, poro=False, solidmode="none"),
+    "3d_thermal_gray": dict(dim=3, n=(12, 10, 9), therm=True, poro=True),
+    "3d_chem": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True),
+    "3d_chem_mixed_solid": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, solidmode="mixed"),
+    "3d_chem_df": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, poro_model="df"),
+    "3d_spherical": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, poro_model="spherical"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_matches_checker(name, hip, oracle64):
+    kw = CASES[name]
+    print(f"[parity fp64] {name} {kw}", flush=True)
+    A = build(hip, prec="fp64", **kw)
+    B = build(oracle64, prec="fp64", **kw)
+    A.init_simulation()
+    B.init_simulation()
+    for nst in (1, 1, 3, 15):
+        A.run(nst)
+        B.run(nst)
+        fa, fb = fields(A), fields(B)
+        for k in fb:
+            assert rel_l2(fa[k], fb[k]) <= 1e-10, (name, A.tLattice, k, rel_l2(fa[k], fb[k]))
+
+
+@pytest.mark.parametrize("name", ["2d_thermal_gray", "3d_thermal_gray", "3d_chem"])
+def test_fp32_against_fp64_checker(name, hip, oracle64):
+    kw = CASES[name]
+    print(f"[parity fp32] {name} {kw}", flush=True)
+    A = build(hip, prec="fp32", **kw)
+    B = build(oracle64, prec="fp64", **kw)
+    A.init_simulation()
+    B.init_simulation()
+    A.run(20)
+    B.run(20)
+    fa, fb = fields(A), fields(B)
+    for k in fb:
+        assert rel_l2(fa[k], fb[k]) <= 1e-5, (name, 20, k, rel_l2(fa[k], fb[k]))
+    A.run(80)
+    B.run(80)
+    fa, fb = fields(A), fields(B)
+    for k in fb:
+        assert rel_l2(fa[k], fb[k]) <= 2e-5, (name, 100, k, rel_l2(fa[k], fb[k]))
+
+
+def test_reductions_match_checker(hip, oracle64):
+    from lbm_porouschem_amd import _abi
+    kw = CASES["3d_chem"]
+    A = build(hip, prec="fp64", **kw)
+    B = build(oracle64, prec="fp64", **kw)
+    A.init_simulation()
+    B.init_simulation()
+    A.run(7)
+    B.run(7)
+    for kind in (_abi.REDUCE_MAX_V, _abi.REDUCE_MIN_TF, _abi.REDUCE_MAX_TF, _abi.REDUCE_MASS, _abi.REDUCE_TF_SUM,
+                 _abi.REDUCE_TS_SUM, _abi.REDUCE_FIX_SUM):
+        a, b = A.reduce(kind), B.reduce(kind)
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (kind, a, b)
+    for j in range(len(A.species)):
+        a, b = A.reduce(_abi.REDUCE_SPECIES_SUM, j), B.reduce(_abi.REDUCE_SPECIES_SUM, j)
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(b))
+    assert A.get_max_v() == pytest.approx(B.get_max_v(), rel=1e-12)
