@@ -23,16 +23,43 @@ def hipcc():
     return cand if os.path.exists(cand) else "hipcc"
 
 
+# translation units: (source, extra flags).  The FP64 kernels are compiled without
+# floating-point contraction so they round like the CPU checker.
+UNITS = [
+    ("lbm_abi.hip", []),
+    ("lbm_f64.hip", ["-ffp-contract=off"]),
+    ("lbm_f32.hip", []),
+]
+
+
+def _errors(stderr):
+    return "\n".join(l for l in (stderr or "").splitlines() if "packed-fp32-ops" not in l)
+
+
 def build(verbose=True):
-    src = os.path.join(CSRC, "lbm_kernels.hip")
+    objdir = os.path.join(CSRC, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    objs = []
+    for src, extra in UNITS:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + extra + ["-c", "-o", obj,
+                                                                             os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        objs.append(obj)
+    for cmd, p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"hipcc failed ({p.returncode}): {' '.join(cmd)}\n{_errors(err)}")
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, src]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
-    errs = "\n".join(l for l in (r.stderr or "").splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{errs}")
+        raise RuntimeError(f"link failed ({r.returncode}):\n{_errors(r.stderr)}")
     os.replace(tmp, OUT)
     return OUT
 

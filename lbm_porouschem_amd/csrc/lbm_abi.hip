@@ -1,0 +1,143 @@
+// lbm_abi.hip -- the C ABI (include/lbm_porouschem.h) over the per-precision
+// implementations in lbm_f64.hip / lbm_f32.hip (kernels in lbm_impl.hpp, per-cell
+// logic in lbm_core.hpp).
+#include "lbm_impl.hpp"
+
+using namespace lbm;
+using namespace lbmimpl;
+
+namespace {
+
+Impl* make_impl(lbm_handle* h) {
+    const lbm_desc& d = h->d;
+    int nf = 0, nfix = 0;
+    if (d.chemical)
+        for (int j = 0; j < d.n_species; ++j) (d.species_fix[j] ? nfix : nf)++;
+    if (d.precision == LBM_PREC_FP64) return make_impl_f64(h, nf, nfix);
+    if (d.precision == LBM_PREC_FP32) return make_impl_f32(h, nf, nfix);
+    return nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+
+size_t lbm_struct_size(int which) { return which == 0 ? sizeof(lbm_desc) : (which == 1 ? sizeof(lbm_params) : 0); }
+
+int lbm_create(const lbm_desc* desc, lbm_handle** out) {
+    if (!desc || !out) return LBM_ERR_INVALID;
+    *out = nullptr;
+    if (desc->abi_version != LBM_ABI_VERSION) return LBM_ERR_INVALID;
+    if (desc->dim != 2 && desc->dim != 3) return LBM_ERR_INVALID;
+    if (desc->nx < 4 || desc->ny < 4 || (desc->dim == 3 && desc->nz < 4) || (desc->dim == 2 && desc->nz != 1))
+        return LBM_ERR_INVALID;
+    if (desc->n_species < 0 || desc->n_species > LBM_MAX_SPECIES) return LBM_ERR_INVALID;
+    lbm_handle* h = new (std::nothrow) lbm_handle();
+    if (!h) return LBM_ERR_OOM;
+    h->d = *desc;
+    h->err[0] = 0;
+    if (desc->collision != LBM_COLL_BGK) {
+        delete h;
+        return LBM_ERR_UNSUPPORTED;
+    }
+    h->impl.reset(make_impl(h));
+    if (!h->impl) {
+        delete h;
+        return LBM_ERR_UNSUPPORTED;
+    }
+    int rc = h->impl->alloc();
+    if (rc) {
+        fprintf(stderr, "lbm_create: %s\n", h->err);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_handle* h) { delete h; }
+
+int lbm_set_params(lbm_handle* h, const lbm_params* p) {
+    if (!h || !p) return LBM_ERR_INVALID;
+    if (p->n_reactions < 0 || p->n_reactions > LBM_MAX_REACTIONS)
+        return set_err(h, LBM_ERR_INVALID, "n_reactions out of range");
+    if (h->d.dim == 3 && p->bc_model != LBM_BCMODEL_NEE && p->bc_model != LBM_BCMODEL_NONE)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "only the NEE boundary model is implemented");
+    if (h->d.dim == 2 && p->bc_model != LBM_BCMODEL_NEE && p->bc_model != LBM_BCMODEL_NONE)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "only the NEE boundary model is implemented");
+    h->p = *p;
+    int rc = h->impl->set_params(*p);
+    if (rc == LBM_OK) h->params_set = 1;
+    return rc;
+}
+
+int lbm_set_field(lbm_handle* h, int field, int comp, const void* host, int dtype, size_t n) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    if (dtype != LBM_DTYPE_F64 && dtype != LBM_DTYPE_F32) return set_err(h, LBM_ERR_INVALID, "dtype %d", dtype);
+    return h->impl->set_field(field, comp, host, dtype, n);
+}
+
+int lbm_get_field(lbm_handle* h, int field, int comp, void* host, int dtype, size_t n) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    if (dtype != LBM_DTYPE_F64 && dtype != LBM_DTYPE_F32) return set_err(h, LBM_ERR_INVALID, "dtype %d", dtype);
+    return h->impl->get_field(field, comp, host, dtype, n);
+}
+
+int lbm_init(lbm_handle* h) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->params_set) return set_err(h, LBM_ERR_STATE, "lbm_set_params must precede lbm_init");
+    h->initialised = 0;
+    int rc = h->impl->init();
+    if (rc) return rc;
+    h->initialised = 1;
+    h->steps = 0;
+    return LBM_OK;
+}
+
+int lbm_step(lbm_handle* h, int nsteps) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->initialised) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_step");
+    if (nsteps < 0) return set_err(h, LBM_ERR_INVALID, "nsteps < 0");
+    if (nsteps == 0) return LBM_OK;
+    int rc = h->impl->step(nsteps);
+    if (rc) return rc;
+    if (h->p.check_divergence) {
+        double vmax = 0;
+        rc = h->impl->reduce(LBM_REDUCE_MAX_V, 0, &vmax);
+        if (rc) return rc;
+        if (!(vmax <= h->p.max_velocity))
+            return set_err(h, LBM_ERR_DIVERGED, "divergence after step %lld (max |u| = %g)", h->steps, vmax);
+    }
+    return LBM_OK;
+}
+
+int lbm_reduce(lbm_handle* h, int kind, int comp, double* out) {
+    if (!h || !out) return LBM_ERR_INVALID;
+    return h->impl->reduce(kind, comp, out);
+}
+
+int lbm_sync(lbm_handle* h) {
+    if (!h) return LBM_ERR_INVALID;
+    return h->impl->sync();
+}
+
+long long lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+int lbm_profile(lbm_handle* h, int enable) {
+    if (!h) return LBM_ERR_INVALID;
+    return h->impl->profile(enable);
+}
+
+int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* launches) {
+    if (!h || !total_ms || !launches) return LBM_ERR_INVALID;
+    return h->impl->profile_read(kernel, total_ms, launches);
+}
+
+const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null handle"; }
+
+}  // extern "C"

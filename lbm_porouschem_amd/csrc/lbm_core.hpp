@@ -187,22 +187,28 @@ LBM_DEV void cell_coord(long long c, int ny, int nz, int& x, int& y, int& z) {
 
 // Index of a cell inside the band of cells at distance < d from any face
 // (d = 2: boundary shell, d = 1: face cells).  Returns -1 outside the band.
+// Three regions, each laid out so that the fastest-varying coordinate of the
+// threads that touch it is contiguous:
+//   A  z-band (3D only)      [zr][x][y]  (z-face kernels run over (x, y))
+//   B  y-band, inner z       [yr][x][z]
+//   C  x-band, inner y and z [xr][y][z]
 template <int D>
 LBM_HD long long band_index(int x, int y, int z, int nx, int ny, int nz, int d) {
     const int dz = D == 3 ? d : 0;
+    const long long nzc = nz - 2 * dz;
     long long A = 2LL * dz * nx * ny;
     if (D == 3 && (z < dz || z >= nz - dz)) {
         int zr = z < dz ? z : z - (nz - 2 * dz);
         return ((long long)zr * nx + x) * ny + y;
     }
-    long long B = 2LL * d * nx * (nz - 2 * dz);
+    long long B = 2LL * d * nx * nzc;
     if (y < d || y >= ny - d) {
         int yr = y < d ? y : y - (ny - 2 * d);
-        return A + ((long long)(z - dz) * 2 * d + yr) * nx + x;
+        return A + ((long long)yr * nx + x) * nzc + (z - dz);
     }
     if (x < d || x >= nx - d) {
         int xr = x < d ? x : x - (nx - 2 * d);
-        return A + B + ((long long)(z - dz) * (ny - 2 * d) + (y - d)) * 2 * d + xr;
+        return A + B + ((long long)xr * (ny - 2 * d) + (y - d)) * nzc + (z - dz);
     }
     return -1;
 }
@@ -214,8 +220,9 @@ LBM_HD long long band_count(int nx, int ny, int nz, int d) {
 template <int D>
 LBM_HD void band_coord(long long i, int nx, int ny, int nz, int d, int& x, int& y, int& z) {
     const int dz = D == 3 ? d : 0;
+    const long long nzc = nz - 2 * dz;
     long long A = 2LL * dz * nx * ny;
-    long long B = 2LL * d * nx * (nz - 2 * dz);
+    long long B = 2LL * d * nx * nzc;
     if (i < A) {
         y = (int)(i % ny);
         long long r = i / ny;
@@ -226,18 +233,18 @@ LBM_HD void band_coord(long long i, int nx, int ny, int nz, int d, int& x, int& 
     }
     i -= A;
     if (i < B) {
-        x = (int)(i % nx);
-        long long r = i / nx;
-        int yr = (int)(r % (2 * d));
-        z = (int)(r / (2 * d)) + dz;
+        z = (int)(i % nzc) + dz;
+        long long r = i / nzc;
+        x = (int)(r % nx);
+        int yr = (int)(r / nx);
         y = yr < d ? yr : yr + (ny - 2 * d);
         return;
     }
     i -= B;
-    int xr = (int)(i % (2 * d));
-    long long r = i / (2 * d);
+    z = (int)(i % nzc) + dz;
+    long long r = i / nzc;
     y = (int)(r % (ny - 2 * d)) + d;
-    z = (int)(r / (ny - 2 * d)) + dz;
+    int xr = (int)(r / (ny - 2 * d));
     x = xr < d ? xr : xr + (nx - 2 * d);
 }
 

@@ -1137,5 +1137,36 @@ int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* lau
     *launches = 0;
     return LBM_OK;
 }
+/* Checker-only accessor (not part of the product ABI): populations of one lattice
+ * after the last step, in the reference's layout.  lattice 0 = f (flow), 1 = TF.g,
+ * 2 = TS.g, 3 + j = g of species j; values of direction k for all cells. */
+int lbm_oracle_pops(lbm_handle* h, int lattice, int k, double* out, size_t n) {
+    if (!h || !out || n != h->N) return LBM_ERR_INVALID;
+    const real* src;
+    int Q;
+    if (lattice == 0) { src = h->f; Q = h->Q; }
+    else if (lattice == 1) { src = h->TF.g; Q = h->Qs; }
+    else if (lattice == 2) { src = h->TS.g; Q = h->Qs; }
+    else {
+        int j = lattice - 3;
+        if (j < 0 || j >= h->d.n_species || is_fix(h, j)) return LBM_ERR_INVALID;
+        src = h->sp[j].g;
+        Q = h->Qs;
+    }
+    if (k < 0 || k >= Q) return LBM_ERR_INVALID;
+    for (size_t i = 0; i < n; ++i) out[i] = (double)src[i * Q + k];
+    return LBM_OK;
+}
+/* Checker-only: velocity set of the flow lattice (e[k][0..2]) and weights. */
+int lbm_oracle_lattice(lbm_handle* h, int scalar, int* e, double* w, int* lr) {
+    if (!h) return LBM_ERR_INVALID;
+    int Q = scalar ? h->Qs : h->Q;
+    for (int k = 0; k < Q; ++k) {
+        for (int a = 0; a < 3; ++a) e[3 * k + a] = scalar ? h->es[k][a] : h->e[k][a];
+        w[k] = scalar ? (double)h->ws[k] : (double)h->w[k];
+        if (lr) lr[k] = h->LR[k];
+    }
+    return Q;
+}
 size_t lbm_struct_size(int which) { return which == 0 ? sizeof(lbm_desc) : (which == 1 ? sizeof(lbm_params) : 0); }
 const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null handle"; }

@@ -1,0 +1,417 @@
+"""Pins the CPU restatement (oracle/lbm_oracle.c) against what the mathematics fixes.
+
+The reference ships no numeric fixture for the step (its tests only check
+constructor shapes, tests/test_lbm2d.py:8-27, tests/test_lbm3d.py:9-30), so each
+part of the oracle is checked here against a closed form, an invariant or a
+textbook property chosen so that a plausible mistake (a dropped term, a wrong
+sign or index, a transposed operand) fails:
+
+  lattice tables        weights/moments of D2Q9, D3Q19, D2Q5, D3Q7; LR = opposite
+  equilibria            zeroth/first/second moments of feq and scalar geq
+  collide + stream      exact mass / scalar conservation in a closed periodic box
+                        with gray and fully solid (bounce-back) cells
+  viscosity, NEE walls  Poiseuille profile in a 2D channel (inlet/outlet/walls)
+  Guo drag + Darcy fix  uniform-flow momentum recursion (derived by hand below)
+  scalar tau            Gaussian pulse variance growth 2*D per step (2D and 3D)
+  interphase exchange   two-temperature relaxation recursion
+  radiation             forward-Euler Stefan-Boltzmann recursion
+  reactions             first-order solid decay S0 (1 - k dt)^(n-1), Arrhenius
+                        k(T) = A (T+1e-6)^b exp(-Ea/(R (T+1e-6))), mass balance,
+                        reaction heat without the 1/v_scale factor (D5)
+  NEE                   sum_k f_b = rho_b after the boundary step
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from lbm_porouschem_amd.flag import BC, BC_FLOW, BC_MODEL, FORCE_TERM, PORO_MODEL, RADIATION_MODEL
+from lbm_porouschem_amd.solver import LBM2DSolver, LBM3DSolver
+
+R = 8.314441
+SIGMA = 5.670374419e-8
+
+
+def _pops(lb, lattice, k):
+    lib = lb._lib.lib
+    f = lib.lbm_oracle_pops
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
+    out = np.empty(lb.nx * lb.ny * lb.nz)
+    assert f(lb._handle, lattice, k, out.ctypes.data, out.size) == 0
+    return out.reshape(lb.nx, lb.ny, lb.nz)
+
+
+def _lattice(lb, scalar):
+    lib = lb._lib.lib
+    f = lib.lbm_oracle_lattice
+    e = (ctypes.c_int * 57)()
+    w = (ctypes.c_double * 19)()
+    lr = (ctypes.c_int * 19)()
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    Q = f(lb._handle, scalar, e, w, lr)
+    return np.array(e[:3 * Q]).reshape(Q, 3), np.array(w[:Q]), np.array(lr[:Q])
+
+
+def _solver(oracle, dim, n, **kw):
+    if dim == 2:
+        return LBM2DSolver(n[0], n[1], 1, dx=1, dt=1, precision="fp64", lib=oracle, **kw)
+    return LBM3DSolver(n[0], n[1], n[2], dx=1, dt=1, precision="fp64", lib=oracle, **kw)
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_tables_and_equilibrium_moments(oracle64, dim):
+    n = (6, 5, 1) if dim == 2 else (5, 4, 4)
+    lb = _solver(oracle64, dim, n[:dim] if dim == 3 else n[:2], isThermal=True)
+    rng = np.random.default_rng(1)
+    shape = (lb.nx, lb.ny, lb.nz)
+    rho = 1 + 0.1 * rng.random(shape)
+    u = 0.05 * (rng.random(shape + (3,)) - 0.5)
+    if dim == 2:
+        u[..., 2] = 0
+    T = rng.random(shape)
+    lb.init_field(lb.rho, rho)
+    lb.v.from_numpy(u)
+    lb.init_field(lb.TF.S, T)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    lb.init_simulation()
+    e, w, lr = _lattice(lb, 0)
+    Q = len(w)
+    cs2 = 1.0 / 3.0
+    assert Q == (9 if dim == 2 else 19)
+    assert np.isclose(w.sum(), 1.0, atol=1e-15)
+    assert np.allclose((w[:, None] * e).sum(0), 0, atol=1e-15)
+    M2 = np.einsum("k,ka,kb->ab", w, e, e)
+    assert np.allclose(M2[:dim, :dim], cs2 * np.eye(dim), atol=1e-15)
+    # fourth-order isotropy of the flow lattice: sum w e_a^2 e_b^2 = cs^4 for a != b, 3 cs^4 for a == b
+    M4 = np.einsum("k,ka,kb->ab", w, e ** 2, e ** 2)
+    assert np.allclose(M4[:dim, :dim], cs2 ** 2 * (np.ones((dim, dim)) + 2 * np.eye(dim)), atol=1e-15)
+    for k in range(Q):
+        assert np.array_equal(e[lr[k]], -e[k])
+    # equilibria: populations right after init are feq(rho, u) (eps = 1)
+    f = np.stack([_pops(lb, 0, k) for k in range(Q)], -1)
+    assert np.allclose(f.sum(-1), rho, rtol=1e-14)
+    mom = np.einsum("...k,ka->...a", f, e.astype(float))
+    assert np.allclose(mom, rho[..., None] * u, atol=1e-15)
+    Pi = np.einsum("...k,ka,kb->...ab", f, e.astype(float), e.astype(float))
+    expect = rho[..., None, None] * (cs2 * np.eye(3) + u[..., :, None] * u[..., None, :])
+    assert np.allclose(Pi[..., :dim, :dim], expect[..., :dim, :dim], atol=1e-14)
+    # scalar lattice (D2Q5 / D3Q7): weights, cs^2, zeroth and first moments of geq
+    es, ws, _ = _lattice(lb, 1)
+    Qs = len(ws)
+    assert Qs == (5 if dim == 2 else 7)
+    assert np.isclose(ws.sum(), 1.0, atol=1e-15)
+    cs2s = 1.0 / 3.0 if dim == 2 else 1.0 / 4.0
+    assert np.allclose(np.einsum("k,ka,kb->ab", ws, es, es)[:dim, :dim], cs2s * np.eye(dim), atol=1e-15)
+    g = np.stack([_pops(lb, 1, k) for k in range(Qs)], -1)
+    assert np.allclose(g.sum(-1), T, rtol=1e-14)
+    assert np.allclose(np.einsum("...k,ka->...a", g, es.astype(float)), T[..., None] * u, atol=1e-15)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mass_and_scalar_conservation_closed_box(oracle64, dim):
+    """No boundary model, periodic wrap, gray and fully solid cells: BGK + Guo + push +
+    bounce-back conserve sum(rho) and sum(TF) exactly (to round-off)."""
+    n = (16, 12) if dim == 2 else (10, 8, 7)
+    lb = _solver(oracle64, dim, n, isThermal=True, isPoro=True)
+    rng = np.random.default_rng(5)
+    shape = (lb.nx, lb.ny, lb.nz)
+    r = rng.random(shape)
+    s = np.where(r < 0.25, 0.4, 0.0) + np.where(r > 0.9, 1.0, 0.0)
+    lb.add_solid(s)
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(0.1)
+    lb.set_poro_Darcy(s > 0, 0.1)
+    lb.set_fluid_Prandtl(0.71)
+    lb.add_rho_solid((s > 0) * 1.0)  # rho_s = 0 under solid > 0 divides by zero in the exchange term (D19)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    rho0 = 1 + 0.05 * rng.random(shape)
+    lb.init_field(lb.rho, rho0)
+    lb.init_field3(lb.v, 0.02 * rng.random(shape), 0.01 * rng.random(shape), 0.0 * rng.random(shape))
+    lb.init_field(lb.TF.S, rng.random(shape))
+    lb.init_simulation()
+    fluid = s < 1
+    m0 = lb.rho.to_numpy()[fluid].sum()
+    t0 = lb.TF.S.to_numpy()[fluid].sum()
+    lb.run(60)
+    m1 = lb.rho.to_numpy()[fluid].sum()
+    t1 = lb.TF.S.to_numpy()[fluid].sum()
+    assert abs(m1 - m0) <= 1e-12 * m0
+    assert abs(t1 - t0) <= 1e-12 * t0
+    # the flow actually moved (not a trivially frozen state)
+    assert np.abs(lb.v.to_numpy()[fluid]).max() > 1e-4
+
+
+def test_poiseuille_channel_2d(oracle64):
+    """Inlet/outlet channel with NEE walls: the developed profile is the parabola
+    u(y) = 4 u_max y (H - y) / H^2 between the wall nodes (H = ny - 1)."""
+    nx, ny = 96, 21
+    lb = _solver(oracle64, 2, (nx, ny))
+    lb.set_viscosity(0.1)
+    lb.set_BC(0, BC_FLOW.inlet)
+    lb.set_v_BC_value(0, [0.01, 0, 0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    lb.set_BC(2, BC_FLOW.wall)
+    lb.set_BC(3, BC_FLOW.wall)
+    lb.init_simulation()
+    lb.run(6000)
+    u = lb.v.to_numpy()[:, :, 0, 0]
+    prof = u[3 * nx // 4]
+    y = np.arange(ny)
+    H = ny - 1
+    shape = 4 * y * (H - y) / H ** 2
+    umax = (prof * shape).sum() / (shape * shape).sum()
+    assert np.abs(prof - umax * shape).max() <= 0.02 * umax
+    # mass flux carried through the channel: inlet column is uniform 0.01 on the ny-2 inner nodes
+    assert abs(prof.sum() - 0.01 * (ny - 2)) <= 0.03 * 0.01 * (ny - 2)
+    assert abs(lb.v.to_numpy()[3 * nx // 4, :, 0, 1]).max() < 1e-3 * umax
+
+
+def _uniform_darcy_recursion(dim, model, rho, u0, sol, nu, cd, steps, cf=0.0):
+    """Hand derivation for spatially uniform fields (streaming is the identity):
+    raw momentum m = sum_k e_k f_k, stored (corrected) velocity u.
+      collision: m' = m - (m - rho u)/tau + (1 - 1/(2 tau)) rho F(u)
+                 (sum_k e_k feq_k = rho u;  sum_k e_k G_k = (1 - 1/(2 tau)) rho F because
+                  the third moment of the weights vanishes)
+      macro:     u' = (m'/rho) / c(u'_raw)
+    with F(u) = -eps nu_F C_D u (DARCY, nu_F dynamic in 3D, kinematic in 2D) or the
+    Ergun form for SPHERICAL, and c the porous velocity correction."""
+    eps = 1 - sol
+    nu_k = nu / rho
+    tau = 3 * nu_k + 0.5
+    nu_F = nu if dim == 3 else nu_k
+    perm = eps ** 3 / (1.000001 - eps) ** 2
+    m = rho * u0.copy()
+    u = u0.copy()
+    for _ in range(steps):
+        if model == "darcy":
+            F = -eps * nu_F * cd * u
+        else:
+            beta = 1.75 / np.sqrt(150 * eps * perm)
+            F = (-nu_F / perm * eps - beta * np.linalg.norm(u)) * u
+        m = m - (m - rho * u) / tau + (1 - 1 / (2 * tau)) * rho * F
+        raw = m / rho
+        if model == "darcy":
+            c = 1 + eps * nu_k * cd / 2
+        else:
+            c0 = 0.5 * (1 + eps * nu_k / 2 * (1.0 / perm))
+            c1 = 0.5 * eps * 1.75 / np.sqrt(150 * eps ** 3)
+            c = c0 + np.sqrt(c0 ** 2 + c1 * np.linalg.norm(raw))
+        u = raw / c
+    return u
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("model", ["darcy", "spherical"])
+def test_uniform_darcy_drag_recursion(oracle64, dim, model):
+    n = (6, 5) if dim == 2 else (5, 4, 4)
+    lb = _solver(oracle64, dim, n, isPoro=True)
+    shape = (lb.nx, lb.ny, lb.nz)
+    sol, cd, nu = 0.35, 0.2, 0.1
+    lb.add_solid(np.full(shape, sol))
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(nu)
+    if model == "darcy":
+        lb.set_poro_Darcy(np.ones(shape), cd)
+    else:
+        lb.poro_model = PORO_MODEL.SPHERICAL
+    lb.boundary_condition_model = BC_MODEL.NONE
+    u0 = np.array([0.02, -0.01, 0.005 if dim == 3 else 0.0])
+    lb.init_field(lb.rho, np.ones(shape))
+    lb.v.from_numpy(np.broadcast_to(u0, shape + (3,)).copy())
+    lb.init_simulation()
+    steps = 25
+    lb.run(steps)
+    u = lb.v.to_numpy()
+    expect = _uniform_darcy_recursion(dim, model, 1.0, u0, sol, nu, cd, steps)
+    assert np.allclose(u, expect, rtol=1e-11, atol=1e-16)
+    assert np.allclose(lb.rho.to_numpy(), 1.0, rtol=1e-14)
+    # the drag really acts: velocity decays
+    assert np.linalg.norm(expect) < 0.9 * np.linalg.norm(u0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gaussian_diffusion_variance(oracle64, dim):
+    """Scalar BGK with D = diff*dt/dx^2: tau = 3D+1/2 (D2Q5) or 4D+1/2 (D3Q7) gives an
+    effective diffusivity D, so the variance of a pulse grows by 2 D per step."""
+    nx = 160
+    n = (nx, 4) if dim == 2 else (nx, 4, 4)
+    lb = _solver(oracle64, dim, n, isThermal=True)
+    lb.set_fluid_thermal_diff(0.15)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    x = np.arange(nx)
+    sig0 = 5.0
+    prof = 0.2 + np.exp(-(x - nx / 2) ** 2 / (2 * sig0 ** 2))
+    shape = (lb.nx, lb.ny, lb.nz)
+    lb.init_field(lb.TF.S, np.broadcast_to(prof[:, None, None], shape).copy())
+    lb.init_simulation()
+    steps = 300
+    lb.run(steps)
+    T = lb.TF.S.to_numpy()[:, 0, 0] - 0.2
+    mass = T.sum()
+    var = ((x - nx / 2) ** 2 * T).sum() / mass
+    D = 0.15
+    # the first collision uses the initial state, so n steps diffuse for n - 1 + 1 = n relaxations
+    assert var == pytest.approx(sig0 ** 2 + 2 * D * steps, rel=5e-3)
+    assert mass == pytest.approx(np.sqrt(2 * np.pi) * sig0, rel=1e-10)
+
+
+def _uniform_thermal_solver(oracle, dim, sol, **kw):
+    n = (5, 4) if dim == 2 else (4, 4, 4)
+    lb = _solver(oracle, dim, n, isThermal=True, isPoro=True, **kw)
+    shape = (lb.nx, lb.ny, lb.nz)
+    lb.add_solid(np.full(shape, sol))
+    lb.set_viscosity(0.1)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    lb.set_fluid_Trange([300.0, 900.0])
+    lb.set_solid_Trange([200.0, 1000.0])
+    lb.set_fluid_capacity(1000.0)
+    lb.set_solid_capacity(1400.0)
+    return lb, shape
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_two_temperature_exchange_recursion(oracle64, dim):
+    """dH = h a (Tf_phys - Ts_phys) dt; TS.dS = dH/(c_s rho_s v_s); TF.dS = -dH/(c_f rho v_f);
+    each macro's source is added by the next collision (sum_k geq_k(dS) = dS)."""
+    lb, shape = _uniform_thermal_solver(oracle64, dim, 0.5)
+    ha, rhos = 0.4, 2.0
+    lb.add_rho_solid(np.full(shape, rhos))
+    lb.set_heat_exchange_coef(np.full(shape, ha))
+    lb.set_heat_exchange_surface(np.ones(shape))
+    lb.init_field(lb.TF.S, np.full(shape, 0.8))
+    lb.init_field(lb.TS.S, np.full(shape, 0.1))
+    lb.init_simulation()
+    steps = 30
+    lb.run(steps)
+    tf, ts = 0.8, 0.1
+    dtf = dts = 0.0
+    for _ in range(steps):
+        tf, ts = tf + dtf, ts + dts
+        dH = ha * 1.0 * ((tf * 600 + 300) - (ts * 800 + 200)) * 1.0
+        dts = dH / 1400.0 / rhos / 800.0
+        dtf = -dH / 1000.0 / 1.0 / 600.0
+    assert np.allclose(lb.TF.S.to_numpy(), tf, rtol=1e-12)
+    assert np.allclose(lb.TS.S.to_numpy(), ts, rtol=1e-12)
+    assert np.allclose(lb.TS.dS.to_numpy(), dts, rtol=1e-10)
+
+
+def test_radiation_forward_euler(oracle64):
+    lb, shape = _uniform_thermal_solver(oracle64, 3, 0.5, isRadiation=True)
+    rhos, emis, area, Ta = 3.0, 0.8, 0.5, 400.0
+    lb.add_rho_solid(np.full(shape, rhos))
+    lb.set_radiation(RADIATION_MODEL.SURFACE_UNIFORM, Ta)
+    lb.init_field(lb.TS.radiation_surface, np.full(shape, area))
+    lb.init_field(lb.TS.emissivity, np.full(shape, emis))
+    lb.init_field(lb.TS.S, np.full(shape, 0.9))
+    lb.init_simulation()
+    steps = 12
+    lb.run(steps)
+    ts, dts = 0.9, 0.0
+    for _ in range(steps):
+        ts = ts + dts
+        T = ts * 800 + 200
+        q = emis * SIGMA * area / 1.0 * (Ta ** 4 - T ** 4)
+        dts = q * 1.0 / 1400.0 / rhos / 800.0
+    assert np.allclose(lb.TS.S.to_numpy(), ts, rtol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_first_order_solid_decay_and_heat(oracle64, dim):
+    """biomass(FIX) => gas with k = A (T+1e-6)^b exp(-Ea/(R (T+1e-6))) at the previous
+    step's solid temperature: FIX density after n steps is S0 (1 - k dt)^(n-1) while T is
+    constant; mass balance dS_gas = -dS_bio for equal molar masses; the reaction heat
+    enters TS.dS as -k dt sum(nu_j h_j)/(c_s rho_s) with no 1/v_scale (D5)."""
+    lb, shape = _uniform_thermal_solver(oracle64, dim, 0.5, isChemical=True)
+    lb.set_specie("bio", True, 50.0)
+    lb.set_specie("gas", False, 50.0)
+    lb.set_specie_diff("gas", 0.1)
+    A, b, Ea = 2.0e-3, 0.5, 9000.0
+    lb.add_reaction("bio => gas", A, Ea, b)
+    lb.set_specie_enthalpy("bio", 0.0)
+    lb.set_specie_enthalpy("gas", 0.0)
+    S0 = 0.7
+    lb.init_specie("bio", np.full(shape, S0))
+    lb.init_specie("gas", np.full(shape, 1.0))
+    lb.init_field(lb.TS.S, np.full(shape, 0.5))
+    lb.init_simulation()
+    steps = 15
+    lb.run(steps)
+    T = 0.5 * 800 + 200
+    k = A * (T + 1e-6) ** b * np.exp(-Ea / (T + 1e-6) / R)
+    M = 50.0 / 1000
+    bio = lb.species[0].S.to_numpy()
+    assert np.allclose(bio, S0 * (1 - k * 1.0) ** (steps - 1), rtol=1e-12)
+    dbio = lb.species[0].dS.to_numpy()
+    dgas = lb.species[1].dS.to_numpy()
+    assert np.allclose(dbio, -k * 1.0 * (bio / M) * M, rtol=1e-12)
+    assert np.allclose(dgas, -dbio, rtol=1e-14)
+
+
+def test_reaction_heat_release(oracle64):
+    lb, shape = _uniform_thermal_solver(oracle64, 3, 0.5, isChemical=True)
+    lb.set_specie("bio", True, 40.0)
+    lb.set_specie("gas", False, 40.0)
+    A = 0.01
+    lb.add_reaction("bio => gas", A, 0.0, 0.0)  # T-independent rate
+    h_bio, h_gas = 5.0e3, -2.0e3
+    lb.set_specie_enthalpy("bio", h_bio)
+    lb.set_specie_enthalpy("gas", h_gas)
+    lb.init_specie("bio", np.full(shape, 0.9))
+    lb.init_specie("gas", np.full(shape, 1.0))
+    lb.init_field(lb.TS.S, np.full(shape, 0.3))
+    lb.init_simulation()
+    steps = 8
+    lb.run(steps)
+    M = 0.04
+    s, ts, dts, ds = 0.9, 0.3, 0.0, 0.0
+    for _ in range(steps):
+        s, ts = s + ds, ts + dts
+        kr = A * (s / M)
+        ds = kr * (-1.0) * 1.0 * M
+        dH = (-1.0 * h_bio + 1.0 * h_gas) * (-kr * 1.0)
+        dts = dH / 1400.0 / s  # rhos = sum of FIX densities, no 1/v_scale
+    assert np.allclose(lb.TS.S.to_numpy(), ts, rtol=1e-12)
+    assert np.allclose(lb.species[0].S.to_numpy(), s, rtol=1e-12)
+
+
+def test_nee_face_population_sum(oracle64):
+    """After the boundary step sum_k f_b = rho_b on every face cell whose neighbour is an
+    interior fluid cell (feq sums to rho and the neighbour's populations sum to rho_nb)."""
+    lb = _solver(oracle64, 3, (10, 8, 7))
+    lb.set_viscosity(0.1)
+    lb.set_BC(0, BC_FLOW.inlet)
+    lb.set_v_BC_value(0, [0.02, 0, 0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    for f in range(2, 6):
+        lb.set_BC(f, BC_FLOW.wall)
+    rng = np.random.default_rng(3)
+    lb.init_field(lb.rho, 1 + 0.01 * rng.random((10, 8, 7)))
+    lb.init_simulation()
+    lb.run(5)
+    f = np.stack([_pops(lb, 0, k) for k in range(19)], -1)
+    rho = lb.rho.to_numpy()
+    face = np.zeros(rho.shape, bool)
+    face[0], face[-1] = True, True
+    face[:, 0], face[:, -1] = True, True
+    face[:, :, 0], face[:, :, -1] = True, True
+    assert np.allclose(f.sum(-1)[face], rho[face], rtol=1e-13)
+    # the outlet face holds the prescribed density, the inlet face the prescribed velocity
+    assert np.allclose(rho[-1, 1:-1, 1:-1], 1.0, rtol=1e-14)
+    assert np.allclose(lb.v.to_numpy()[0, 1:-1, 1:-1, 0], 0.02, rtol=1e-14)
+
+
+def test_fp32_build_tracks_fp64(oracle64, oracle32):
+    from cases import build, fields, rel_l2
+    kw = dict(dim=3, n=(10, 8, 7), therm=True, poro=True, chem=True)
+    A = build(oracle32, prec="fp32", **kw)
+    B = build(oracle64, prec="fp64", **kw)
+    A.init_simulation()
+    B.init_simulation()
+    A.run(10)
+    B.run(10)
+    fa, fb = fields(A), fields(B)
+    for k in fb:
+        assert rel_l2(fa[k], fb[k]) < 2e-5, k
