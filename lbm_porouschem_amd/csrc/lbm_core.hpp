@@ -475,7 +475,11 @@ LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real 
         Real kr = (Real)0.0;
         if (P.thermal) {
             Real T = solidT ? phys(P.ts, Tlag) : phys(P.tf, Tlag);
-            if (T > R.Tmin) kr = R.A * (Real)pow(T + (Real)1e-6, R.b) * (Real)exp(-R.Ea / (T + (Real)1e-6) / (Real)GAS_R);
+            if (T > R.Tmin) {
+                const Real Tg = T + (Real)1e-6;
+                // pow(x, 0) == 1 exactly, so the common b = 0 case skips the pow
+                kr = R.A * (R.b == (Real)0 ? (Real)1 : (Real)pow(Tg, R.b)) * (Real)exp(-R.Ea / Tg / (Real)GAS_R);
+            }
         } else {
             kr = R.A;
         }
@@ -534,9 +538,10 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
         rho += F;
     }
     if (P.eos == EOS_IDEAL) {
-        ux /= rho;
-        uy /= rho;
-        uz /= rho;
+        const Real ir = 1 / rho;
+        ux *= ir;
+        uy *= ir;
+        uz *= ir;
     }
     const Real eps = m.eps;
     if (P.poro && eps != 0 && eps != 1) {
@@ -563,9 +568,10 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
             c = (Real)1;
         }
         if (P.poro_model <= PORO_DF) {
-            ux /= c;
-            uy /= c;
-            uz /= c;
+            const Real ic = 1 / c;
+            ux *= ic;
+            uy *= ic;
+            uz *= ic;
         }
     }
     m.rho = rho;
@@ -593,9 +599,10 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
 #pragma unroll
         for (int j = 0; j < NX; ++j)
             if (j < P.nfix && P.poro) rhos += m.S[NF + j];
+        const Real iY = 1 / Yall;
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (j < P.nf) m.S[j] /= Yall;
+            if (j < P.nf) m.S[j] *= iY;
         if (P.poro) m.rhos = rhos;
         reactions<Real, NF, NX>(P, m, Tlag, Tlag);
     }
@@ -654,6 +661,14 @@ template <class Real> LBM_DEV Real sp_D(const DevParams<Real>& P, int j, Real TF
 // ---------------------------------------------------------------------------
 // collision + sources of one cell (LBM3D/_evolution.py:31-66), in place on p.
 // FIX species densities are advanced by dS (m.S[NF+j] += m.dS[NF+j]).
+//
+// The reference's per-direction expressions are regrouped around per-cell
+// constants (same mathematics, divisions hoisted to one reciprocal per cell):
+//   flow   f* = (1 - 1/tau) f + w_k [ b0 + b1 eu + b2 eu^2 + g3 eF + g9 eu eF ]
+//          with feq_k = w_k (a0 + a1 eu + a2 eu^2), a2 = 4.5 rho/(eps+1e-12) (ideal gas),
+//          Guo_k = w_k (1 - 1/(2 tau)) rho [3 (e - u/eps').F + 9 (e.u)(e.F)/eps']
+//   scalar g* = (1 - 1/tau_s) g + w_k P_k (S/tau_s + dS),  P_k = geq_k / (w_k S)
+//          (equilibrium and source share P_k because both use the cell velocity)
 // ---------------------------------------------------------------------------
 template <class Real, int D, int NF, int NX>
 LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, bool pts) {
@@ -664,6 +679,7 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
     const Real kin = kin_from_dyn(P, visc, rho);
     const Real tau = 3 * kin + (Real)0.5;
     const Real itau = 1 / tau;
+    const Real omf = 1 - itau;
     // Guo forcing of the gray-LBM Darcy drag (darcy_force / forceTermGuo, :263-293)
     Real Fx = 0, Fy = 0, Fz = 0;
     const bool guo = P.force_guo != 0;
@@ -674,7 +690,7 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
         bool any = true;
         if (P.poro_model == PORO_SPHERICAL) {
             Real a = (Real)1.000001 - e;
-            Real perm = (Real)pow(e, (Real)3) / (a * a) * 1;
+            Real perm = e * e * e / (a * a);
             Real beta = (Real)1.75 / (Real)sqrt((Real)150 * e * perm);
             c = -nu / perm * e - beta * (Real)sqrt(ux * ux + uy * uy + uz * uz);
         } else if (P.poro_model == PORO_DARCY || (D == 2 && P.poro_model == PORO_DARCY_HIGH)) {
@@ -690,30 +706,48 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
             Fz = c * uz;
         }
     }
-    const Real ee = (P.eos == EOS_INCOMP && D == 2) ? eps : eps + (Real)1e-12;
-    const Real gpre = P.eos == EOS_INCOMP ? ((Real)1.0 - (Real)1.0 / (Real)2.0 / tau)
-                                          : ((Real)1.0 - (Real)1.0 / (Real)2.0 / tau) * rho;
-    const Real uox = ux / ee, uoy = uy / ee, uoz = uz / ee;
+    const bool incomp = P.eos == EOS_INCOMP;
+    const Real ee = (incomp && D == 2) ? eps : eps + (Real)1e-12;
+    const Real iee = 1 / ee;
+    const Real uv = ux * ux + uy * uy + uz * uz;
+    // feq_k / w_k = a0 + a1 eu + a2 eu^2
+    const Real a0 = incomp ? rho - (Real)1.5 * uv * iee : rho * (1 - (Real)1.5 * uv * iee);
+    const Real a1 = incomp ? (Real)3 : 3 * rho;
+    const Real a2 = incomp ? (Real)4.5 * iee : (Real)4.5 * rho * iee;
+    const Real gpre = guo ? (incomp ? (1 - (Real)0.5 * itau) : (1 - (Real)0.5 * itau) * rho) : (Real)0;
+    const Real g3 = 3 * gpre, g9 = 9 * gpre * iee;
+    const Real uoF = (ux * Fx + uy * Fy + uz * Fz) * iee;
+    const Real b0 = itau * a0 - g3 * uoF, b1 = itau * a1, b2 = itau * a2;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        const Real fe = feq<D, Real>(P.eos, k, rho, ux, uy, uz, eps);
-        Real f = p.f[k] - itau * (p.f[k] - fe);
-        if (guo) {
-            const Real ex = (Real)L::ex(k), ey = (Real)L::ey(k), ez = (Real)L::ez(k);
-            const Real emF = (ex - uox) * Fx + (ey - uoy) * Fy + (ez - uoz) * Fz;
-            const Real eu = ex * ux + ey * uy + ez * uz;
-            const Real eF = ex * Fx + ey * Fy + ez * Fz;
-            f += gpre * (Real)L::w(k) * ((Real)3.0 * emF + (Real)9.0 * eu * eF / ee);
+        const int ex = L::ex(k), ey = L::ey(k), ez = L::ez(k);
+        Real eu = 0, eF = 0;
+        if (ex) { eu += ex * ux; eF += ex * Fx; }
+        if (ey) { eu += ey * uy; eF += ey * Fy; }
+        if (ez) { eu += ez * uz; eF += ez * Fz; }
+        const Real poly = b0 + eu * (b1 + b2 * eu) + eF * (g3 + g9 * eu);
+        p.f[k] = omf * p.f[k] + (Real)L::w(k) * poly;
+    }
+    // scalar equilibria share P_k: 3D 1 + 4eu + 8eu^2 - 2u^2 (TF and species),
+    // 2D TF 1 + 3eu + 4.5eu^2 - 1.5u^2, 2D species 1 + 3eu (linear)
+    Real Pt[QS], Ps[QS];
+#pragma unroll
+    for (int k = 0; k < QS; ++k) {
+        const Real eu = (Real)L::ex(k) * ux + (Real)L::ey(k) * uy + (Real)L::ez(k) * uz;
+        if (D == 3) {
+            Pt[k] = (Real)L::ws(k) * (1 + 4 * eu + 8 * eu * eu - 2 * uv);
+            Ps[k] = Pt[k];
+        } else {
+            Pt[k] = (Real)L::ws(k) * (1 + 3 * eu + (Real)4.5 * eu * eu - (Real)1.5 * uv);
+            Ps[k] = (Real)L::ws(k) * (1 + 3 * eu);
         }
-        p.f[k] = f;
     }
     if (P.thermal) {
         const Real itT = 1 / scalar_tau<D, Real>(tf_D<Real, NF, NX>(P, m, kin));
+        const Real src = itT * m.TF + m.dTF;
+        const Real om = 1 - itT;
 #pragma unroll
-        for (int k = 0; k < QS; ++k) {
-            Real g = p.gT[k] - itT * (p.gT[k] - geq_tf<D, Real>(k, m.TF, ux, uy, uz));
-            p.gT[k] = g + geq_tf<D, Real>(k, m.dTF, ux, uy, uz);
-        }
+        for (int k = 0; k < QS; ++k) p.gT[k] = om * p.gT[k] + Pt[k] * src;
     }
     // FIX species: S += dS at direction 0; with a mixture solid conductivity the
     // TS relaxation time of directions >= 1 sees the updated densities.
@@ -730,10 +764,8 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
                    : itS0;
 #pragma unroll
         for (int k = 0; k < QS; ++k) {
-            const Real w = (Real)L::ws(k);
             const Real it = k == 0 ? itS0 : itS1;
-            Real g = p.gS[k] - it * (p.gS[k] - w * m.TS);
-            p.gS[k] = g + w * m.dTS;
+            p.gS[k] = (1 - it) * p.gS[k] + (Real)L::ws(k) * (it * m.TS + m.dTS);
         }
     }
     if (P.chem) {
@@ -741,12 +773,10 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
         for (int j = 0; j < NF; ++j) {
             if (j < P.nf) {
                 const Real it = 1 / scalar_tau<D, Real>(sp_D(P, j, m.TF));
+                const Real src = it * m.S[j] + m.dS[j];
+                const Real om = 1 - it;
 #pragma unroll
-                for (int k = 0; k < QS; ++k) {
-                    Real g = p.gP[j][k] - it * (p.gP[j][k] - geq_sp<D, Real>(k, m.S[j], ux, uy, uz));
-                    // 3D adds TF.geq7(dS) (same formula as the species equilibrium); 2D the linear geq5
-                    p.gP[j][k] = g + (D == 3 ? geq_tf<D, Real>(k, m.dS[j], ux, uy, uz) : geq_sp<D, Real>(k, m.dS[j], ux, uy, uz));
-                }
+                for (int k = 0; k < QS; ++k) p.gP[j][k] = om * p.gP[j][k] + Ps[k] * src;
             }
         }
     }
@@ -764,14 +794,56 @@ LBM_DEV void collide_emitter(const DevParams<Real>& P, Real* g, const CellMac<Re
 // ---------------------------------------------------------------------------
 // A-A population access
 // ---------------------------------------------------------------------------
+// Offsets inside one lattice are element offsets relative to the lattice's slot 0
+// (slot * N + cell).  For 4-byte and 2-byte populations they are 32-bit (the host
+// checks that a lattice spans < 4 GiB), so every access is an SGPR base plus one
+// 32-bit VGPR offset; FP64 lattices use 64-bit offsets.
+template <class Store> struct OffT { using type = uint32_t; };
+template <> struct OffT<double> { using type = unsigned long long; };
+
+template <class Store>
+LBM_DEV const Store* at(const Store* base, typename OffT<Store>::type elem) {
+    return (const Store*)((const char*)base + (typename OffT<Store>::type)(elem * (typename OffT<Store>::type)sizeof(Store)));
+}
+template <class Store>
+LBM_DEV Store* at(Store* base, typename OffT<Store>::type elem) {
+    return (Store*)((char*)base + (typename OffT<Store>::type)(elem * (typename OffT<Store>::type)sizeof(Store)));
+}
+
+// Cell coordinates and branch-free periodic neighbour offsets
+// (LBM3D/_evolution.py:201-210: a step off one face re-enters at the opposite one).
 struct Geo {
     long long c;
     int x, y, z, nx, ny, nz;
     bool edge;
+    int sx, sy;                    // strides of x and y (z is contiguous)
+    int wxp, wxm, wyp, wym, wzp, wzm;  // wrap corrections added when e_a = +1 / -1
 };
+template <int D>
+LBM_DEV void geo_init(Geo& g, long long c, int nx, int ny, int nz) {
+    g.c = c;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    cell_coord(c, ny, nz, g.x, g.y, g.z);
+    g.sx = ny * nz;
+    g.sy = nz;
+    g.wxp = g.x == nx - 1 ? -nx * g.sx : 0;
+    g.wxm = g.x == 0 ? nx * g.sx : 0;
+    g.wyp = g.y == ny - 1 ? -ny * g.sy : 0;
+    g.wym = g.y == 0 ? ny * g.sy : 0;
+    g.wzp = (D == 3 && g.z == nz - 1) ? -nz : 0;
+    g.wzm = (D == 3 && g.z == 0) ? nz : 0;
+    g.edge = g.x == 0 || g.x == nx - 1 || g.y == 0 || g.y == ny - 1 || (D == 3 && (g.z == 0 || g.z == nz - 1));
+}
 template <int D> LBM_DEV long long nb_of(const Geo& g, int k) {
     using L = Lat<D>;
-    return nbr<D>(g.c, g.x, g.y, g.z, L::ex(k), L::ey(k), L::ez(k), g.nx, g.ny, g.nz, g.edge);
+    const int ex = L::ex(k), ey = L::ey(k), ez = L::ez(k);
+    long long off = (long long)ex * g.sx + (long long)ey * g.sy + ez;
+    off += ex > 0 ? g.wxp : (ex < 0 ? g.wxm : 0);
+    off += ey > 0 ? g.wyp : (ey < 0 ? g.wym : 0);
+    off += ez > 0 ? g.wzp : (ez < 0 ? g.wzm : 0);
+    return g.c + off;
 }
 
 // gather the incoming populations of one lattice (QN directions, slots base..base+QN-1)
@@ -780,17 +852,21 @@ template <int D, int QN, int PAR, class Real, class Store>
 LBM_DEV void gather(const Store* __restrict__ pops, long long N, int base, const Geo& g, uint32_t mask, int bitbase,
                     Real* out) {
     using L = Lat<D>;
+    using O = typename OffT<Store>::type;
+    const Store* lb = pops + (long long)base * N;
+    const O c = (O)g.c, n = (O)N;
 #pragma unroll
     for (int k = 0; k < QN; ++k) {
-        long long src;
+        O e;
         if (PAR == 0) {
-            src = (long long)(base + k) * N + g.c;
+            e = (O)k * n + c;
         } else {
             const int kr = L::lr(k);
-            if ((mask >> (bitbase + kr)) & 1u) src = (long long)(base + kr) * N + nb_of<D>(g, kr);
-            else src = (long long)(base + k) * N + g.c;
+            const O far = (O)kr * n + (O)nb_of<D>(g, kr);
+            const O own = (O)k * n + c;
+            e = ((mask >> (bitbase + kr)) & 1u) ? far : own;
         }
-        out[k] = Codec<Store, Real>::load(pops + src, (Real)0);
+        out[k] = Codec<Store, Real>::load(at(lb, e), (Real)0);
     }
 }
 // scatter post-collision populations with the same parity rule
@@ -798,29 +874,37 @@ template <int D, int QN, int PAR, class Real, class Store>
 LBM_DEV void scatter(Store* __restrict__ pops, long long N, int base, const Geo& g, uint32_t mask, int bitbase,
                      const Real* in) {
     using L = Lat<D>;
+    using O = typename OffT<Store>::type;
+    Store* lb = pops + (long long)base * N;
+    const O c = (O)g.c, n = (O)N;
 #pragma unroll
     for (int k = 0; k < QN; ++k) {
-        long long dst;
         const int kr = L::lr(k);
+        O e;
         if (PAR == 0) {
-            dst = (long long)(base + kr) * N + g.c;
+            e = (O)kr * n + c;
         } else {
-            if ((mask >> (bitbase + k)) & 1u) dst = (long long)(base + k) * N + nb_of<D>(g, k);
-            else dst = (long long)(base + kr) * N + g.c;
+            const O far = (O)k * n + (O)nb_of<D>(g, k);
+            const O own = (O)kr * n + c;
+            e = ((mask >> (bitbase + k)) & 1u) ? far : own;
         }
-        Codec<Store, Real>::store(pops + dst, in[k], (Real)0);
+        Codec<Store, Real>::store(at(lb, e), in[k], (Real)0);
     }
 }
 // fixed slots (lattice not evolved at this cell): slot k holds direction k
 template <int QN, class Real, class Store>
 LBM_DEV void load_fixed(const Store* __restrict__ pops, long long N, int base, long long c, Real* out) {
+    using O = typename OffT<Store>::type;
+    const Store* lb = pops + (long long)base * N;
 #pragma unroll
-    for (int k = 0; k < QN; ++k) out[k] = Codec<Store, Real>::load(pops + (long long)(base + k) * N + c, (Real)0);
+    for (int k = 0; k < QN; ++k) out[k] = Codec<Store, Real>::load(at(lb, (O)k * (O)N + (O)c), (Real)0);
 }
 template <int QN, class Real, class Store>
 LBM_DEV void store_fixed(Store* __restrict__ pops, long long N, int base, long long c, const Real* in) {
+    using O = typename OffT<Store>::type;
+    Store* lb = pops + (long long)base * N;
 #pragma unroll
-    for (int k = 0; k < QN; ++k) Codec<Store, Real>::store(pops + (long long)(base + k) * N + c, in[k], (Real)0);
+    for (int k = 0; k < QN; ++k) Codec<Store, Real>::store(at(lb, (O)k * (O)N + (O)c), in[k], (Real)0);
 }
 
 // side buffer (shell cells), same slot numbering as pops
@@ -983,12 +1067,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     const long long N = P.N;
     const uint32_t mask = B.mask[c];
     Geo g;
-    g.c = c;
-    g.nx = P.nx;
-    g.ny = P.ny;
-    g.nz = P.nz;
-    cell_coord(c, P.ny, P.nz, g.x, g.y, g.z);
-    g.edge = (mask & FACE_BIT) != 0;
+    geo_init<D>(g, c, P.nx, P.ny, P.nz);
     const bool shell = (mask & SHELL_BIT) != 0;
     const bool face = (mask & FACE_BIT) != 0;
     const bool pflow = (mask & 1u) != 0;
@@ -1171,12 +1250,9 @@ LBM_DEV void face_collide_cell(const DevParams<Real>& P, const Bufs<Real, Store>
     constexpr int QS = L::QS;
     const long long N = P.N;
     Geo g;
-    g.nx = P.nx;
-    g.ny = P.ny;
-    g.nz = P.nz;
-    band_coord<D>(fi, P.nx, P.ny, P.nz, 1, g.x, g.y, g.z);
-    g.c = ((long long)g.x * P.ny + g.y) * P.nz + g.z;
-    g.edge = true;
+    int fx, fy, fz;
+    band_coord<D>(fi, P.nx, P.ny, P.nz, 1, fx, fy, fz);
+    geo_init<D>(g, ((long long)fx * P.ny + fy) * P.nz + fz, P.nx, P.ny, P.nz);
     const long long c = g.c;
     const uint32_t mask = B.mask[c];
     const bool pflow = (mask & 1u) != 0;
@@ -1221,12 +1297,7 @@ LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     const long long N = P.N;
     const uint32_t mask = B.mask[c];
     Geo g;
-    g.c = c;
-    g.nx = P.nx;
-    g.ny = P.ny;
-    g.nz = P.nz;
-    cell_coord(c, P.ny, P.nz, g.x, g.y, g.z);
-    g.edge = (mask & FACE_BIT) != 0;
+    geo_init<D>(g, c, P.nx, P.ny, P.nz);
     const bool pflow = (mask & 1u) != 0;
     const bool pts = ((mask >> TS_BIT) & 1u) != 0;
     CellPops<Real, D, NF, NX> p;
@@ -1289,14 +1360,9 @@ template <class Real, int D>
 LBM_DEV uint32_t make_mask(const Real* __restrict__ solid, long long c, int nx, int ny, int nz) {
     using L = Lat<D>;
     Geo g;
-    g.c = c;
-    g.nx = nx;
-    g.ny = ny;
-    g.nz = nz;
-    cell_coord(c, ny, nz, g.x, g.y, g.z);
-    const bool face = g.x == 0 || g.x == nx - 1 || g.y == 0 || g.y == ny - 1 || (D == 3 && (g.z == 0 || g.z == nz - 1));
+    geo_init<D>(g, c, nx, ny, nz);
+    const bool face = g.edge;
     const bool shell = g.x <= 1 || g.x >= nx - 2 || g.y <= 1 || g.y >= ny - 2 || (D == 3 && (g.z <= 1 || g.z >= nz - 2));
-    g.edge = face;
     uint32_t m = 0;
 #pragma unroll
     for (int k = 0; k < L::Q; ++k)
