@@ -64,6 +64,11 @@ constexpr int TS_BIT = 19;
 constexpr uint32_t FACE_BIT = 1u << 26;   // distance 0 from a face
 constexpr uint32_t SHELL_BIT = 1u << 27;  // distance <= 1 from a face
 constexpr uint32_t SOLIDNZ_BIT = 1u << 28; // solid fraction != 0 (the kernel must read it)
+// fast-axis band cell (z in 3D, y in 2D; within 1 of the face) of a deep row whose
+// fast-face NEE runs inside the bulk kernel: the face cell and its inner neighbour
+// sit in adjacent lanes of one wavefront, so the neighbour's populations and
+// moments are exchanged with lane shuffles instead of a trip through memory
+constexpr uint32_t ZFAST_BIT = 1u << 29;
 
 // ---------------------------------------------------------------------------
 // velocity sets: LBM3D/_initialize.py:25-43, LBM2D/_initialize.py:24-35
@@ -162,6 +167,8 @@ template <class Real, class Store> struct Bufs {
     Real* S;              // [nf+nfix][N]
     Real* dS;             // [nf+nfix][N]
     Real* rhos;
+    const uint32_t* flist;  // face cells whose NEE + collision run in the face kernels
+    long long nflist;
     Real* vprof;          // face velocity profiles, 3 values per face cell, faces concatenated
     Real* rprof;          // face density profiles
     long long prof_off[6];
@@ -187,65 +194,91 @@ LBM_DEV void cell_coord(long long c, int ny, int nz, int& x, int& y, int& z) {
 
 // Index of a cell inside the band of cells at distance < d from any face
 // (d = 2: boundary shell, d = 1: face cells).  Returns -1 outside the band.
-// Three regions, each laid out so that the fastest-varying coordinate of the
-// threads that touch it is contiguous:
-//   A  z-band (3D only)      [zr][x][y]  (z-face kernels run over (x, y))
-//   B  y-band, inner z       [yr][x][z]
-//   C  x-band, inner y and z [xr][y][z]
+// The fastest-varying axis (z in 3D, y in 2D) is the "fast" axis; a "row" is a
+// line of cells along it.  Two regions:
+//   R1  perimeter rows (a slow coordinate within d of a face): whole rows,
+//       [row][fast], so a row of threads touches contiguous memory;
+//   R2  fast-axis band cells of the deep rows: [fr][deep row].
 template <int D>
 LBM_HD long long band_index(int x, int y, int z, int nx, int ny, int nz, int d) {
-    const int dz = D == 3 ? d : 0;
-    const long long nzc = nz - 2 * dz;
-    long long A = 2LL * dz * nx * ny;
-    if (D == 3 && (z < dz || z >= nz - dz)) {
-        int zr = z < dz ? z : z - (nz - 2 * dz);
-        return ((long long)zr * nx + x) * ny + y;
+    if (D == 3) {
+        const bool inY = y < d || y >= ny - d;
+        const bool inX = x < d || x >= nx - d;
+        const long long nr1 = 2LL * d * nx + 2LL * d * (ny - 2 * d);
+        if (inY) {
+            const int yr = y < d ? y : y - (ny - 2 * d);
+            return ((long long)yr * nx + x) * nz + z;
+        }
+        if (inX) {
+            const int xr = x < d ? x : x - (nx - 2 * d);
+            return (2LL * d * nx + (long long)xr * (ny - 2 * d) + (y - d)) * nz + z;
+        }
+        if (z < d || z >= nz - d) {
+            const int zr = z < d ? z : z - (nz - 2 * d);
+            const long long ndeep = (long long)(nx - 2 * d) * (ny - 2 * d);
+            return nr1 * nz + (long long)zr * ndeep + (long long)(x - d) * (ny - 2 * d) + (y - d);
+        }
+        return -1;
+    } else {
+        const long long nr1 = 2LL * d;
+        if (x < d || x >= nx - d) {
+            const int xr = x < d ? x : x - (nx - 2 * d);
+            return (long long)xr * ny + y;
+        }
+        if (y < d || y >= ny - d) {
+            const int yr = y < d ? y : y - (ny - 2 * d);
+            return nr1 * ny + (long long)yr * (nx - 2 * d) + (x - d);
+        }
+        return -1;
     }
-    long long B = 2LL * d * nx * nzc;
-    if (y < d || y >= ny - d) {
-        int yr = y < d ? y : y - (ny - 2 * d);
-        return A + ((long long)yr * nx + x) * nzc + (z - dz);
-    }
-    if (x < d || x >= nx - d) {
-        int xr = x < d ? x : x - (nx - 2 * d);
-        return A + B + ((long long)xr * (ny - 2 * d) + (y - d)) * nzc + (z - dz);
-    }
-    return -1;
 }
 template <int D>
 LBM_HD long long band_count(int nx, int ny, int nz, int d) {
-    const int dz = D == 3 ? d : 0;
-    return 2LL * dz * nx * ny + 2LL * d * nx * (nz - 2 * dz) + 2LL * d * (ny - 2 * d) * (nz - 2 * dz);
+    if (D == 3) {
+        const long long nr1 = 2LL * d * nx + 2LL * d * (ny - 2 * d);
+        return nr1 * nz + 2LL * d * (nx - 2 * d) * (ny - 2 * d);
+    }
+    return 2LL * d * ny + 2LL * d * (nx - 2 * d);
 }
 template <int D>
 LBM_HD void band_coord(long long i, int nx, int ny, int nz, int d, int& x, int& y, int& z) {
-    const int dz = D == 3 ? d : 0;
-    const long long nzc = nz - 2 * dz;
-    long long A = 2LL * dz * nx * ny;
-    long long B = 2LL * d * nx * nzc;
-    if (i < A) {
-        y = (int)(i % ny);
-        long long r = i / ny;
-        x = (int)(r % nx);
-        int zr = (int)(r / nx);
-        z = zr < dz ? zr : zr + (nz - 2 * dz);
-        return;
-    }
-    i -= A;
-    if (i < B) {
-        z = (int)(i % nzc) + dz;
-        long long r = i / nzc;
-        x = (int)(r % nx);
-        int yr = (int)(r / nx);
+    if (D == 3) {
+        const long long nr1 = 2LL * d * nx + 2LL * d * (ny - 2 * d);
+        if (i < nr1 * nz) {
+            z = (int)(i % nz);
+            const long long row = i / nz;
+            if (row < 2LL * d * nx) {
+                x = (int)(row % nx);
+                const int yr = (int)(row / nx);
+                y = yr < d ? yr : yr + (ny - 2 * d);
+            } else {
+                const long long r = row - 2LL * d * nx;
+                y = (int)(r % (ny - 2 * d)) + d;
+                const int xr = (int)(r / (ny - 2 * d));
+                x = xr < d ? xr : xr + (nx - 2 * d);
+            }
+            return;
+        }
+        i -= nr1 * nz;
+        const long long ndeep = (long long)(nx - 2 * d) * (ny - 2 * d);
+        const int zr = (int)(i / ndeep);
+        const long long r = i % ndeep;
+        x = (int)(r / (ny - 2 * d)) + d;
+        y = (int)(r % (ny - 2 * d)) + d;
+        z = zr < d ? zr : zr + (nz - 2 * d);
+    } else {
+        z = 0;
+        if (i < 2LL * d * ny) {
+            y = (int)(i % ny);
+            const int xr = (int)(i / ny);
+            x = xr < d ? xr : xr + (nx - 2 * d);
+            return;
+        }
+        i -= 2LL * d * ny;
+        const int yr = (int)(i / (nx - 2 * d));
+        x = (int)(i % (nx - 2 * d)) + d;
         y = yr < d ? yr : yr + (ny - 2 * d);
-        return;
     }
-    i -= B;
-    z = (int)(i % nzc) + dz;
-    long long r = i / nzc;
-    y = (int)(r % (ny - 2 * d)) + d;
-    int xr = (int)(r / (ny - 2 * d));
-    x = xr < d ? xr : xr + (nx - 2 * d);
 }
 
 // neighbour c + e (periodic wrap as LBM3D/_evolution.py:201-210)
@@ -1056,6 +1089,78 @@ LBM_DEV void scatter_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, c
 }
 
 // ---------------------------------------------------------------------------
+// NEE of a fast-axis face cell from its inner neighbour's state (LBM3D/_boundary.py:
+// 260-282 for z-faces, LBM2D/_boundary.py:80-101 for y-faces; scalar part
+// LBM3D/_scalarField.py:123-141).  p/m hold the face cell's gathered populations and
+// pre-BC moments on entry and its post-BC populations and moments on return; n*
+// are the neighbour's populations, moments and solid fraction.
+// ---------------------------------------------------------------------------
+template <class Real, class Store, int D, int NF, int NX>
+LBM_DEV void nee_inwave(const DevParams<Real>& P, const Bufs<Real, Store>& B, int face, long long fc,
+                        CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, const CellPops<Real, D, NF, NX>& pn,
+                        Real nrho, Real nux, Real nuy, Real nuz, Real nTF, Real nTS, const Real* nS, Real nsol) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q, QS = L::QS;
+    Real ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
+    const int bv = P.bc_v[face], br = P.bc_rho[face];
+    if (bv == BC_FIXED) {
+        const Real* vp = B.vprof + B.prof_off[face] * 3 + fc * 3;
+        ux = vp[0];
+        uy = vp[1];
+        uz = vp[2];
+    } else if (bv == BC_ZEROGRAD) {
+        ux = nux;
+        uy = nuy;
+        uz = nuz;
+    }
+    if (br == BC_FIXED) rho = P.rho_bc[face];
+    else if (br == BC_ZEROGRAD && !(D == 3 && face == 4)) rho = nrho;
+    const Real eb = 1 - m.sol, enb = 1 - nsol;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        p.f[k] = feq<D, Real>(P.eos, k, rho, ux, uy, uz, eb) + (pn.f[k] - feq<D, Real>(P.eos, k, nrho, nux, nuy, nuz, enb));
+    if (P.chem) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            if (j < P.nf) {
+                const SpecP<Real>& sp = P.sp[j];
+                Real Sb = m.S[j];
+                if (sp.bc_type[face] == BC_FIXED) Sb = sp.bc_value[face];
+                if (sp.bc_type[face] == BC_ZEROGRAD) Sb = nS[j];
+#pragma unroll
+                for (int k = 0; k < QS; ++k)
+                    p.gP[j][k] = geq_sp<D, Real>(k, Sb, ux, uy, uz) + (pn.gP[j][k] - geq_sp<D, Real>(k, nS[j], nux, nuy, nuz));
+                m.S[j] = Sb;
+            }
+        }
+    }
+    if (P.thermal) {
+        Real Tb = m.TF;
+        if (P.tf.bc_type[face] == BC_FIXED) Tb = P.tf.bc_value[face];
+        if (P.tf.bc_type[face] == BC_ZEROGRAD) Tb = nTF;
+#pragma unroll
+        for (int k = 0; k < QS; ++k)
+            p.gT[k] = geq_tf<D, Real>(k, Tb, ux, uy, uz) + (pn.gT[k] - geq_tf<D, Real>(k, nTF, nux, nuy, nuz));
+        m.TF = Tb;
+        Real Sb = m.TS;
+        if (P.ts.bc_type[face] == BC_FIXED) Sb = P.ts.bc_value[face];
+        if (P.ts.bc_type[face] == BC_ZEROGRAD) Sb = nTS;
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            const Real w = (Real)L::ws(k);
+            p.gS[k] = w * Sb + (pn.gS[k] - w * nTS);
+        }
+        m.TS = Sb;
+    }
+    m.rho = rho;
+    m.ux = ux;
+    m.uy = uy;
+    m.uz = uz;
+}
+
+template <class T> LBM_DEV T lane_fetch(T v, int src) { return __shfl(v, src, 64); }
+
+// ---------------------------------------------------------------------------
 // kernel bodies
 // ---------------------------------------------------------------------------
 // Bulk: gather (streaming of collision n) -> macro n -> [shell: record] ->
@@ -1076,6 +1181,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
 
     CellPops<Real, D, NF, NX> p;
     CellMac<Real, NF, NX> m;
+    const bool zfast = (mask & ZFAST_BIT) != 0;
     if (!pflow) {
         // fully solid cell: flow, TF and species are frozen; TS relaxes from its own populations
         if (shell) {
@@ -1110,7 +1216,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     if (P.thermal) {
         gather<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
         if (pts) gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
-        else if (shell || write_macro) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+        else if (shell || write_macro || zfast) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
         else {
 #pragma unroll
             for (int k = 0; k < QS; ++k) p.gS[k] = (Real)0;
@@ -1135,6 +1241,46 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
 #pragma unroll
     for (int j = 0; j < NF; ++j) Sold[j] = (P.lag_sp && j < P.nf) ? B.lagS[(long long)j * N + c] : (Real)0;
     macro<Real, D, NF, NX>(P, p, m, Tlag, TFold, Sold);
+
+    // fast-axis face cells of deep rows: NEE from the neighbouring lane, then proceed
+    // like an interior cell (collision from the post-BC state)
+    const int fpos = D == 3 ? g.z : g.y;
+    const int nfast = D == 3 ? P.nz : P.ny;
+    const bool zface = zfast && (fpos == 0 || fpos == nfast - 1);
+    if (__ballot(zface) != 0) {
+        const int lane = (int)(threadIdx.x & 63);
+        const int src = lane + (zface ? (fpos == 0 ? 1 : -1) : 0);
+        CellPops<Real, D, NF, NX> pn;
+#pragma unroll
+        for (int k = 0; k < L::Q; ++k) pn.f[k] = lane_fetch(p.f[k], src);
+        Real nTF = 0, nTS = 0;
+        if (P.thermal) {
+#pragma unroll
+            for (int k = 0; k < QS; ++k) {
+                pn.gT[k] = lane_fetch(p.gT[k], src);
+                pn.gS[k] = lane_fetch(p.gS[k], src);
+            }
+            nTF = lane_fetch(m.TF, src);
+            nTS = lane_fetch(m.TS, src);
+        }
+        Real nS[NF > 0 ? NF : 1];
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            nS[j] = (Real)0;
+            if (P.chem && j < P.nf) {
+#pragma unroll
+                for (int k = 0; k < QS; ++k) pn.gP[j][k] = lane_fetch(p.gP[j][k], src);
+                nS[j] = lane_fetch(m.S[j], src);
+            }
+        }
+        const Real nrho = lane_fetch(m.rho, src), nux = lane_fetch(m.ux, src), nuy = lane_fetch(m.uy, src),
+                   nuz = lane_fetch(m.uz, src), nsol = lane_fetch(m.sol, src);
+        if (zface) {
+            const int face = D == 3 ? (fpos == 0 ? 4 : 5) : (fpos == 0 ? 2 : 3);
+            const long long fc = D == 3 ? (long long)g.x * P.ny + g.y : (long long)g.x;
+            nee_inwave<Real, Store, D, NF, NX>(P, B, face, fc, p, m, pn, nrho, nux, nuy, nuz, nTF, nTS, nS, nsol);
+        }
+    }
 
     if (shell) {
         side_put_all<Real, Store, D, NF, NX>(P, B, si, p);
@@ -1250,9 +1396,7 @@ LBM_DEV void face_collide_cell(const DevParams<Real>& P, const Bufs<Real, Store>
     constexpr int QS = L::QS;
     const long long N = P.N;
     Geo g;
-    int fx, fy, fz;
-    band_coord<D>(fi, P.nx, P.ny, P.nz, 1, fx, fy, fz);
-    geo_init<D>(g, ((long long)fx * P.ny + fy) * P.nz + fz, P.nx, P.ny, P.nz);
+    geo_init<D>(g, (long long)B.flist[fi], P.nx, P.ny, P.nz);
     const long long c = g.c;
     const uint32_t mask = B.mask[c];
     const bool pflow = (mask & 1u) != 0;
@@ -1370,8 +1514,21 @@ LBM_DEV uint32_t make_mask(const Real* __restrict__ solid, long long c, int nx, 
 #pragma unroll
     for (int k = 0; k < L::QS; ++k)
         if (solid[nb_of<D>(g, k)] > 0) m |= 1u << (TS_BIT + k);
-    if (face) m |= FACE_BIT;
-    if (shell) m |= SHELL_BIT;
+    // rows along the fast axis that are at least 2 cells from every slow-axis face
+    const int nfast = D == 3 ? nz : ny;
+    const int fpos = D == 3 ? g.z : g.y;
+    const bool deep = D == 3 ? (g.x >= 2 && g.x <= nx - 3 && g.y >= 2 && g.y <= ny - 3) : (g.x >= 2 && g.x <= nx - 3);
+    bool zfast = false;
+    if (deep && nfast % 64 == 0 && nfast >= 64 && (fpos < 2 || fpos >= nfast - 2)) {
+        const long long row = c - fpos;
+        zfast = solid[row] < 1 && solid[row + 1] < 1 && solid[row + nfast - 2] < 1 && solid[row + nfast - 1] < 1;
+    }
+    if (zfast) {
+        m |= ZFAST_BIT;
+    } else {
+        if (face) m |= FACE_BIT;
+        if (shell) m |= SHELL_BIT;
+    }
     if (solid[c] != 0) m |= SOLIDNZ_BIT;
     return m;
 }

@@ -73,12 +73,15 @@ __global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict
     } else if (family == 1) {
         if (t >= (long long)nx * nz) return;
         int x = (int)(t / nz), z = (int)(t % nz);
+        // 2D: the y faces are the fast-axis faces; rows handled inside the bulk kernel are skipped
+        if (D == 2 && (B.mask[(long long)x * ny] & ZFAST_BIT)) return;
         long long fc = (long long)x * nz + z;
         nee_cell<Real, Store, D, NF, NX>(P, B, 2, x, 0, z, x, 1, z, fc);
         nee_cell<Real, Store, D, NF, NX>(P, B, 3, x, ny - 1, z, x, ny - 2, z, fc);
     } else {
         if (t >= (long long)nx * ny) return;
         int x = (int)(t / ny), y = (int)(t % ny);
+        if (B.mask[((long long)x * ny + y) * nz] & ZFAST_BIT) return;
         long long fc = (long long)x * ny + y;
         nee_cell<Real, Store, D, NF, NX>(P, B, 4, x, y, 0, x, y, 1, fc);
         nee_cell<Real, Store, D, NF, NX>(P, B, 5, x, y, nz - 1, x, y, nz - 2, fc);
@@ -89,7 +92,7 @@ template <class Real, class Store, int D, int NF, int NX, int PAR>
 __global__ __launch_bounds__(BLOCK) void k_face(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B) {
     const DevParams<Real>& P = *Pp;
     long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= P.nface) return;
+    if (t >= B.nflist) return;
     face_collide_cell<Real, Store, D, NF, NX, PAR>(P, B, t);
 }
 
@@ -241,6 +244,8 @@ struct ImplT final : Impl {
     Real *exch_coef = nullptr, *exch_surf = nullptr, *rad_surf = nullptr, *emis = nullptr, *ha = nullptr, *rad = nullptr;
     Real* solid = nullptr;
     double* partials = nullptr;
+    uint32_t* flist = nullptr;
+    long long nflist = 0;
     bool has_emit = false;
     // device-event timing per kernel class
     bool prof_on = false;
@@ -303,6 +308,8 @@ struct ImplT final : Impl {
     }
 
     int alloc() override {
+        if (N >= (1LL << 32) || (sizeof(Store) <= 4 && (unsigned long long)Q * N * sizeof(Store) >= (1ULL << 32)))
+            return set_err(h_, LBM_ERR_UNSUPPORTED, "lattice of %lld cells exceeds the 32-bit population offsets", N);
         HIP_TRY(hipSetDevice(h_->d.device));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         const int therm = h_->d.thermal, chem = h_->d.chemical;
@@ -606,6 +613,30 @@ struct ImplT final : Impl {
         hipLaunchKernelGGL((k_mask<Real, D>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, solid, (uint32_t*)B.mask, N,
                            nx, ny, nz);
         HIP_TRY(hipGetLastError());
+        // face cells left to the face kernels: every face cell except the fast-axis
+        // face cells that the bulk kernel handles in-wave
+        {
+            std::vector<uint32_t> mh((size_t)N);
+            HIP_TRY(hipMemcpyAsync(mh.data(), B.mask, N * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            std::vector<uint32_t> fl;
+            fl.reserve((size_t)nface);
+            for (long long i = 0; i < nface; ++i) {
+                int x, y, z;
+                band_coord<D>(i, nx, ny, nz, 1, x, y, z);
+                const long long c = ((long long)x * ny + y) * nz + z;
+                if (mh[(size_t)c] & FACE_BIT) fl.push_back((uint32_t)c);
+            }
+            nflist = (long long)fl.size();
+            if (!flist) {
+                int rc = dalloc(&flist, (size_t)nface);
+                if (rc) return rc;
+            }
+            if (nflist)
+                HIP_TRY(hipMemcpyAsync(flist, fl.data(), fl.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+            B.flist = flist;
+            B.nflist = nflist;
+        }
         // inlet profile of face 0 (init_boundary, LBM3D/_initialize.py:79-88)
         int inflow = p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW;
         Real v0 = (Real)(p.flow_bc[0] / ny / nz);
@@ -651,10 +682,14 @@ struct ImplT final : Impl {
             }
             ev_end(1, e_nee);
             hipEvent_t e_face = ev_begin(2);
-            if (stepno & 1)
-                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1>), dim3(grid_for(nface)), dim3(BLOCK), 0, stream, dP, B);
-            else
-                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0>), dim3(grid_for(nface)), dim3(BLOCK), 0, stream, dP, B);
+            if (nflist) {
+                if (stepno & 1)
+                    hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1>), dim3(grid_for(nflist)), dim3(BLOCK), 0, stream,
+                                       dP, B);
+                else
+                    hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0>), dim3(grid_for(nflist)), dim3(BLOCK), 0, stream,
+                                       dP, B);
+            }
             ev_end(2, e_face);
             ev_end(3, e_step);
             HIP_TRY(hipGetLastError());

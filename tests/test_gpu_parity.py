@@ -22,6 +22,11 @@ CASES = {
     "3d_chem_mixed_solid": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, solidmode="mixed"),
     "3d_chem_df": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, poro_model="df"),
     "3d_spherical": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, poro_model="spherical"),
+    # fast axis a multiple of 64: the z-face (2D: y-face) boundary runs inside the bulk kernel
+    "3d_fastface_chem": dict(dim=3, n=(9, 8, 64), therm=True, poro=True, chem=True),
+    "3d_fastface_mixed_solid": dict(dim=3, n=(9, 8, 64), therm=True, poro=True, chem=True, solidmode="mixed"),
+    "2d_fastface_thermal": dict(dim=2, n=(20, 64), therm=True, poro=True),
+    "2d_fastface_mixed_solid": dict(dim=2, n=(20, 128), therm=True, poro=True, solidmode="mixed"),
 }
 
 
@@ -41,7 +46,7 @@ def test_fp64_matches_checker(name, hip, oracle64):
             assert rel_l2(fa[k], fb[k]) <= 1e-10, (name, A.tLattice, k, rel_l2(fa[k], fb[k]))
 
 
-@pytest.mark.parametrize("name", ["2d_thermal_gray", "3d_thermal_gray", "3d_chem"])
+@pytest.mark.parametrize("name", ["2d_thermal_gray", "3d_thermal_gray", "3d_chem", "3d_fastface_chem"])
 def test_fp32_against_fp64_checker(name, hip, oracle64):
     kw = CASES[name]
     print(f"[parity fp32] {name} {kw}", flush=True)
