@@ -58,6 +58,33 @@ enum { DIFF_NONE = 0, DIFF_CONSTANT = 1, DIFF_SCHMIDT = 2 };
 enum { EOS_IDEAL = 0, EOS_INCOMP = 1 };
 enum { BC_PERIODIC = 0, BC_ZEROGRAD = 1, BC_FIXED = 2 };
 
+// ---------------------------------------------------------------------------
+// Model policies.  Each member is a model switch of the reference (the enums of
+// src/LBM/util/flag.py plus module flags); -1 means "read it from DevParams at run
+// time", a value >= 0 fixes it at compile time so the kernel carries no code,
+// branches or live registers for the other models.  ModelAny serves every
+// configuration; ModelCoupled is the 3D coupled pyrolysis setup (BGK, ideal gas,
+// Guo-forced Darcy drag, constant viscosity, Prandtl fluid diffusivity, constant
+// solid diffusivity and capacities, constant species diffusivities, no radiation)
+// with exactly NF fluid and NX FIX species.  The host selects ModelCoupled only when
+// every switch of the parameters matches.
+// ---------------------------------------------------------------------------
+struct ModelAny {
+    static constexpr int thermal = -1, chem = -1, poro = -1, radiation = -1, force_guo = -1, eos = -1, visc = -1,
+                         poro_model = -1, tf_diff = -1, tf_cond = -1, tf_cap = -1, ts_diff = -1, ts_cond = -1,
+                         ts_cap = -1, sp_visc = -1, sp_diff = -1, sp_cond = -1, lag_tf = -1, lag_sp = -1,
+                         exact_species = 0;
+};
+struct ModelCoupled {
+    static constexpr int thermal = 1, chem = 1, poro = 1, radiation = 0, force_guo = 1, eos = EOS_IDEAL,
+                         visc = VISC_CONSTANT, poro_model = PORO_DARCY, tf_diff = TDIFF_PRANDTL, tf_cond = -1,
+                         tf_cap = THERMO_CONSTANT, ts_diff = TDIFF_CONSTANT, ts_cond = -1, ts_cap = THERMO_CONSTANT,
+                         sp_visc = -1, sp_diff = DIFF_CONSTANT, sp_cond = -1, lag_tf = 0, lag_sp = 0,
+                         exact_species = 1;
+};
+#define LBM_MV(key, runtime) (M::key >= 0 ? M::key : (runtime))
+
+
 // mask bits per cell: bit k (k < Q): flow lattice evolves at neighbour c+e_k (solid < 1);
 // bit TS_BIT+k (k < QS): TS evolves at neighbour c+e_k (solid > 0).  Bit 0 / TS_BIT are the cell itself.
 constexpr int TS_BIT = 19;
@@ -302,121 +329,121 @@ template <class Real> LBM_DEV Real poly5_ref(const Real* c, Real T) {
 }
 
 // Specie.viscosity (LBM3D/_chemical.py:70-77); TFlu = TF.S in lattice units
-template <class Real> LBM_DEV Real sp_visc(const DevParams<Real>& P, int j, Real TFlu) {
+template <class Real, class M = ModelAny> LBM_DEV Real sp_visc(const DevParams<Real>& P, int j, Real TFlu) {
     const SpecP<Real>& s = P.sp[j];
     Real visco = (Real)1e-5;
-    if (s.visc_model == VISC_CONSTANT) {
+    if (LBM_MV(sp_visc, s.visc_model) == VISC_CONSTANT) {
         visco = s.visc_lu;
-    } else if (s.visc_model == VISC_SUTHERLAND) {
-        Real T = P.thermal ? phys(P.tf, TFlu) : (Real)273.15;
+    } else if (LBM_MV(sp_visc, s.visc_model) == VISC_SUTHERLAND) {
+        Real T = LBM_MV(thermal, P.thermal) ? phys(P.tf, TFlu) : (Real)273.15;
         visco = s.suth0 * (Real)pow(T, (Real)1.5) / (T + s.suth1);
     }
     return visco;
 }
 
 // viscosity(i) with the TF.S / species S values the caller decides on
-template <class Real, int NF>
+template <class Real, int NF, class M = ModelAny>
 LBM_DEV Real visc_dyn(const DevParams<Real>& P, Real TFlu, const Real* Sfl) {
     Real visco = (Real)0.1;
-    if (P.visc_model == VISC_CONSTANT) {
+    if (LBM_MV(visc, P.visc_model) == VISC_CONSTANT) {
         visco = P.visc_lu;
-    } else if (P.visc_model == VISC_SUTHERLAND) {
-        Real T = P.thermal ? phys(P.tf, TFlu) : (Real)273.15;
+    } else if (LBM_MV(visc, P.visc_model) == VISC_SUTHERLAND) {
+        Real T = LBM_MV(thermal, P.thermal) ? phys(P.tf, TFlu) : (Real)273.15;
         visco = P.suth0 * (Real)pow(T, (Real)1.5) / (T + P.suth1) * P.dtdx2;
-    } else if (P.visc_model == VISC_MIXTURE) {
-        if (P.chem) {
+    } else if (LBM_MV(visc, P.visc_model) == VISC_MIXTURE) {
+        if (LBM_MV(chem, P.chem)) {
 #pragma unroll
             for (int j = 0; j < NF; ++j)
-                if (j < P.nf) visco += Sfl[j] * sp_visc(P, j, TFlu);
+                if (j < (M::exact_species ? NF : P.nf)) visco += Sfl[j] * sp_visc<Real, M>(P, j, TFlu);
         }
     }
     return visco;
 }
-template <class Real> LBM_DEV Real kin_from_dyn(const DevParams<Real>& P, Real visc, Real rho) {
-    return P.eos == EOS_IDEAL ? visc / rho : visc;
+template <class Real, class M = ModelAny> LBM_DEV Real kin_from_dyn(const DevParams<Real>& P, Real visc, Real rho) {
+    return LBM_MV(eos, P.eos) == EOS_IDEAL ? visc / rho : visc;
 }
 
-template <class Real> LBM_DEV Real sp_cond(const DevParams<Real>& P, int j, Real TFlu) {
+template <class Real, class M = ModelAny> LBM_DEV Real sp_cond(const DevParams<Real>& P, int j, Real TFlu) {
     const SpecP<Real>& s = P.sp[j];
     Real cond = (Real)0.0;
-    if (s.cond_model == COND_CONSTANT) {
+    if (LBM_MV(sp_cond, s.cond_model) == COND_CONSTANT) {
         cond += s.cond;
-    } else if (s.cond_model == COND_POLY) {
-        Real T = P.thermal ? phys(P.tf, TFlu) : (Real)273.15;
+    } else if (LBM_MV(sp_cond, s.cond_model) == COND_POLY) {
+        Real T = LBM_MV(thermal, P.thermal) ? phys(P.tf, TFlu) : (Real)273.15;
         const Real* c = s.cond_poly;
         cond += (((c[4] * T + c[3]) * T + c[2]) * T + c[1]) * T + c[0];
     }
     return cond;
 }
 
-template <class Real, int NF>
+template <class Real, int NF, class M = ModelAny>
 LBM_DEV Real tf_cond(const DevParams<Real>& P, Real TFlu, const Real* S) {
     const ThermP<Real>& t = P.tf;
     Real kappa = (Real)0.0;
-    if (t.cond_model == COND_CONSTANT) kappa = t.cond;
-    else if (t.cond_model == COND_POLY) kappa = poly5_ref(t.cond_poly, phys(t, TFlu));
-    else if (t.cond_model == COND_MIXTURE) {
-        if (P.chem) {
+    if (LBM_MV(tf_cond, t.cond_model) == COND_CONSTANT) kappa = t.cond;
+    else if (LBM_MV(tf_cond, t.cond_model) == COND_POLY) kappa = poly5_ref(t.cond_poly, phys(t, TFlu));
+    else if (LBM_MV(tf_cond, t.cond_model) == COND_MIXTURE) {
+        if (LBM_MV(chem, P.chem)) {
 #pragma unroll
             for (int j = 0; j < NF; ++j)
-                if (j < P.nf) kappa += sp_cond(P, j, TFlu) * S[j];
+                if (j < (M::exact_species ? NF : P.nf)) kappa += sp_cond<Real, M>(P, j, TFlu) * S[j];
         }
     }
     return kappa;
 }
-template <class Real, int NF>
+template <class Real, int NF, class M = ModelAny>
 LBM_DEV Real tf_cap(const DevParams<Real>& P, Real TFlu, const Real* S) {
     const ThermP<Real>& t = P.tf;
     Real cm = (Real)0.0;
-    if (t.cap_model == THERMO_CONSTANT) cm = t.cm;
-    else if (t.cap_model == THERMO_POLY) cm = poly5_ref(t.cm_poly, phys(t, TFlu));
-    else if (t.cap_model == THERMO_NASA7) {
+    if (LBM_MV(tf_cap, t.cap_model) == THERMO_CONSTANT) cm = t.cm;
+    else if (LBM_MV(tf_cap, t.cap_model) == THERMO_POLY) cm = poly5_ref(t.cm_poly, phys(t, TFlu));
+    else if (LBM_MV(tf_cap, t.cap_model) == THERMO_NASA7) {
         Real T = phys(t, TFlu);
         cm = T < t.T1 ? poly5_ref(t.nasa[0], T) : poly5_ref(t.nasa[1], T);
         cm *= (Real)GAS_R;
-    } else if (t.cap_model == THERMO_MIXTURE) {
-        if (P.chem) {
+    } else if (LBM_MV(tf_cap, t.cap_model) == THERMO_MIXTURE) {
+        if (LBM_MV(chem, P.chem)) {
 #pragma unroll
             for (int j = 0; j < NF; ++j)
-                if (j < P.nf) cm += P.sp[j].cap_m * S[j];
+                if (j < (M::exact_species ? NF : P.nf)) cm += P.sp[j].cap_m * S[j];
         }
     }
     return cm;
 }
 // TemperatureSolid.conductivity: MIXTURE sums FIX species S_j/rhos
-template <class Real, int NF, int NX>
+template <class Real, int NF, int NX, class M = ModelAny>
 LBM_DEV Real ts_cond(const DevParams<Real>& P, Real TSlu, Real TFlu, const Real* S, Real rhos) {
     const ThermP<Real>& t = P.ts;
     Real kappa = (Real)0.0;
-    if (t.cond_model == COND_CONSTANT) kappa = t.cond;
-    else if (t.cond_model == COND_POLY) {
+    if (LBM_MV(ts_cond, t.cond_model) == COND_CONSTANT) kappa = t.cond;
+    else if (LBM_MV(ts_cond, t.cond_model) == COND_POLY) {
         Real T = phys(t, TSlu);
         const Real* c = t.cond_poly;
         kappa = c[0] + T * (c[1] + T * (c[2] * T + T * (c[3])));
-    } else if (t.cond_model == COND_MIXTURE) {
-        if (P.chem) {
+    } else if (LBM_MV(ts_cond, t.cond_model) == COND_MIXTURE) {
+        if (LBM_MV(chem, P.chem)) {
 #pragma unroll
             for (int j = 0; j < NX; ++j)
-                if (j < P.nfix) kappa += sp_cond(P, NF + j, TFlu) * S[NF + j] / rhos;
+                if (j < (M::exact_species ? NX : P.nfix)) kappa += sp_cond<Real, M>(P, NF + j, TFlu) * S[NF + j] / rhos;
         }
     }
     return kappa;
 }
 // TemperatureSolid.capacity_m: NASA7 without *R; MIXTURE sums fluid species * rho (D18)
-template <class Real, int NF>
+template <class Real, int NF, class M = ModelAny>
 LBM_DEV Real ts_cap(const DevParams<Real>& P, Real TSlu, const Real* S, Real rho) {
     const ThermP<Real>& t = P.ts;
     Real cm = (Real)0.0;
-    if (t.cap_model == THERMO_CONSTANT) cm = t.cm;
-    else if (t.cap_model == THERMO_POLY) cm = poly5_ref(t.cm_poly, phys(t, TSlu));
-    else if (t.cap_model == THERMO_NASA7) {
+    if (LBM_MV(ts_cap, t.cap_model) == THERMO_CONSTANT) cm = t.cm;
+    else if (LBM_MV(ts_cap, t.cap_model) == THERMO_POLY) cm = poly5_ref(t.cm_poly, phys(t, TSlu));
+    else if (LBM_MV(ts_cap, t.cap_model) == THERMO_NASA7) {
         Real T = phys(t, TSlu);
         cm = T < t.T1 ? poly5_ref(t.nasa[0], T) : poly5_ref(t.nasa[1], T);
-    } else if (t.cap_model == THERMO_MIXTURE) {
-        if (P.chem) {
+    } else if (LBM_MV(ts_cap, t.cap_model) == THERMO_MIXTURE) {
+        if (LBM_MV(chem, P.chem)) {
 #pragma unroll
             for (int j = 0; j < NF; ++j)
-                if (j < P.nf) cm += P.sp[j].cap_m * S[j] * rho;
+                if (j < (M::exact_species ? NF : P.nf)) cm += P.sp[j].cap_m * S[j] * rho;
         } else {
             cm += 1000;
         }
@@ -495,7 +522,7 @@ template <class Real, int NF, int NX> struct CellMac {
 // ---------------------------------------------------------------------------
 // reactions (LBM3D/_chemical.py:189-270), lagged temperature Tlag (lattice units)
 // ---------------------------------------------------------------------------
-template <class Real, int NF, int NX>
+template <class Real, int NF, int NX, class M = ModelAny>
 LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real Tlag, Real TFold) {
     constexpr int NS = NF + NX;
     Real acc[NS > 0 ? NS : 1];
@@ -506,7 +533,7 @@ LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real 
     for (int r = 0; r < P.n_reac; ++r) {
         const ReacP<Real>& R = P.r[r];
         Real kr = (Real)0.0;
-        if (P.thermal) {
+        if (LBM_MV(thermal, P.thermal)) {
             Real T = solidT ? phys(P.ts, Tlag) : phys(P.tf, Tlag);
             if (T > R.Tmin) {
                 const Real Tg = T + (Real)1e-6;
@@ -518,7 +545,7 @@ LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real 
         }
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            const bool live = j < NF ? j < P.nf : (j - NF) < P.nfix;
+            const bool live = j < NF ? j < (M::exact_species ? NF : P.nf) : (j - NF) < (M::exact_species ? NX : P.nfix);
             if (live && R.reac[j] > 0) {
                 if (m.S[j] > P.tol) {
                     if (R.rate[j] != 0) {
@@ -533,55 +560,54 @@ LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real 
         Real dH = (Real)0.0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            const bool live = j < NF ? j < P.nf : (j - NF) < P.nfix;
+            const bool live = j < NF ? j < (M::exact_species ? NF : P.nf) : (j - NF) < (M::exact_species ? NX : P.nfix);
             Real coef = -R.reac[j] + R.prod[j];
             if (live && coef != (Real)0.) {
                 acc[j] += kr * coef * P.dt * P.sp[j].molemass;
                 dH += coef * P.sp[j].enthalpy;
             }
         }
-        if (P.thermal) accH += dH * (-kr * P.dt);
+        if (LBM_MV(thermal, P.thermal)) accH += dH * (-kr * P.dt);
     }
 #pragma unroll
     for (int j = 0; j < NS; ++j) m.dS[j] = acc[j];
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         // capacity with the previous step's temperature (update_dS runs before the T update)
-        if (m.sol > 0) m.dTS += accH / ts_cap<Real, NF>(P, Tlag, m.S, m.rho) / m.rhos;
-        else m.dTF += accH / tf_cap<Real, NF>(P, TFold, m.S) / m.rho;
+        if (m.sol > 0) m.dTS += accH / ts_cap<Real, NF, M>(P, Tlag, m.S, m.rho) / m.rhos;
+        else m.dTF += accH / tf_cap<Real, NF, M>(P, TFold, m.S) / m.rho;
     }
 }
 
 // ---------------------------------------------------------------------------
-// macro (LBM3D/_evolution.py:118-196) from gathered populations.
-// Tlag: lagged reaction temperature; TFold: TF.S of the previous step (porous
-// viscosity models); Sold: previous species S (mixture viscosity).
+// flow half of macro (LBM3D/_evolution.py:118-150): density, velocity and the
+// gray-LBM porous velocity correction
 // ---------------------------------------------------------------------------
-template <class Real, int D, int NF, int NX>
-LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m,
-                   Real Tlag, Real TFold, const Real* Sold) {
+template <class Real, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void macro_flow(const DevParams<Real>& P, const Real* __restrict__ f, CellMac<Real, NF, NX>& m, Real TFold,
+                        const Real* Sold) {
     using L = Lat<D>;
-    constexpr int Q = L::Q, QS = L::QS;
+    constexpr int Q = L::Q;
     Real rho = (Real)0.0, ux = (Real)0.0, uy = (Real)0.0, uz = (Real)0.0;
 #pragma unroll
     for (int s = 0; s < Q; ++s) {
-        Real F = p.f[s];
+        Real F = f[s];
         if (L::ex(s) == 1) ux += F; else if (L::ex(s) == -1) ux += -F;
         if (L::ey(s) == 1) uy += F; else if (L::ey(s) == -1) uy += -F;
         if (L::ez(s) == 1) uz += F; else if (L::ez(s) == -1) uz += -F;
         rho += F;
     }
-    if (P.eos == EOS_IDEAL) {
+    if (LBM_MV(eos, P.eos) == EOS_IDEAL) {
         const Real ir = 1 / rho;
         ux *= ir;
         uy *= ir;
         uz *= ir;
     }
     const Real eps = m.eps;
-    if (P.poro && eps != 0 && eps != 1) {
-        Real visc = visc_dyn<Real, NF>(P, TFold, Sold);
-        Real kin = kin_from_dyn(P, visc, rho);
+    if (LBM_MV(poro, P.poro) && eps != 0 && eps != 1) {
+        Real visc = visc_dyn<Real, NF, M>(P, TFold, Sold);
+        Real kin = kin_from_dyn<Real, M>(P, visc, rho);
         Real c;
-        if (P.poro_model == PORO_SPHERICAL) {
+        if (LBM_MV(poro_model, P.poro_model) == PORO_SPHERICAL) {
             Real a = (Real)1.000001 - eps;
             Real perm = (Real)pow(eps, (Real)3) / (a * a) * 1;
             Real Dc = (Real)1.0 / perm;
@@ -590,9 +616,9 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
             Real c1 = (Real)0.5 * eps * Fc;
             Real vn = (Real)sqrt(ux * ux + uy * uy + uz * uz);
             c = c0 + (Real)sqrt(c0 * c0 + c1 * vn);
-        } else if (P.poro_model == PORO_DARCY) {
+        } else if (LBM_MV(poro_model, P.poro_model) == PORO_DARCY) {
             c = 1 + eps * kin * m.cd_ / 2;
-        } else if (P.poro_model == PORO_DF) {
+        } else if (LBM_MV(poro_model, P.poro_model) == PORO_DF) {
             Real c0 = (Real)0.5 * (1 + eps * kin / 2 * m.cd_);
             Real c1 = (Real)0.5 * eps * m.cf_;
             Real vn = (Real)sqrt(ux * ux + uy * uy + uz * uz);
@@ -600,7 +626,7 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
         } else {
             c = (Real)1;
         }
-        if (P.poro_model <= PORO_DF) {
+        if (LBM_MV(poro_model, P.poro_model) <= PORO_DF) {
             const Real ic = 1 / c;
             ux *= ic;
             uy *= ic;
@@ -611,16 +637,25 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
     m.ux = ux;
     m.uy = uy;
     m.uz = uz;
+}
+
+// scalar half of macro: species sums and normalisation, solid density,
+// reactions with the lagged temperature, temperatures, interphase exchange, radiation
+template <class Real, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void macro_scalar(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m,
+                          Real Tlag) {
+    using L = Lat<D>;
+    constexpr int QS = L::QS;
     m.TF = (Real)0.0;
     m.TS = (Real)0.0;
     m.dTF = (Real)0.0;
     m.dTS = (Real)0.0;
-    if (P.chem) {
+    if (LBM_MV(chem, P.chem)) {
         Real rhos = (Real)0.0;
         Real Yall = (Real)0.0;
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
-            if (j < P.nf) {
+            if (j < (M::exact_species ? NF : P.nf)) {
                 Real S = (Real)0.0;
 #pragma unroll
                 for (int s = 0; s < QS; ++s) S += p.gP[j][s];
@@ -631,15 +666,15 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
         }
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (j < P.nfix && P.poro) rhos += m.S[NF + j];
+            if (j < (M::exact_species ? NX : P.nfix) && LBM_MV(poro, P.poro)) rhos += m.S[NF + j];
         const Real iY = 1 / Yall;
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (j < P.nf) m.S[j] *= iY;
-        if (P.poro) m.rhos = rhos;
-        reactions<Real, NF, NX>(P, m, Tlag, Tlag);
+            if (j < (M::exact_species ? NF : P.nf)) m.S[j] *= iY;
+        if (LBM_MV(poro, P.poro)) m.rhos = rhos;
+        reactions<Real, NF, NX, M>(P, m, Tlag, Tlag);
     }
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         Real Tf = (Real)0.0, Ts = (Real)0.0;
 #pragma unroll
         for (int s = 0; s < QS; ++s) {
@@ -650,44 +685,54 @@ LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p,
         m.TS = Ts;
         if (m.sol > 0) {
             Real dH = m.ha_ * (phys(P.tf, Tf) - phys(P.ts, Ts)) * P.dt;
-            m.dTS += dH / ts_cap<Real, NF>(P, Ts, m.S, m.rho) / m.rhos / P.ts.v_scale;
-            m.dTF += -dH / tf_cap<Real, NF>(P, Tf, m.S) / m.rho / P.tf.v_scale;
+            m.dTS += dH / ts_cap<Real, NF, M>(P, Ts, m.S, m.rho) / m.rhos / P.ts.v_scale;
+            m.dTF += -dH / tf_cap<Real, NF, M>(P, Tf, m.S) / m.rho / P.tf.v_scale;
         }
-        if (P.radiation && P.rad_model == 1) {
+        if (LBM_MV(radiation, P.radiation) && P.rad_model == 1) {
             Real T = phys(P.ts, Ts);
             Real q = (Real)0.0;
             q += m.rad_ * (P.t_amb4 - (Real)pow(T, (Real)4));
-            m.dTS += q * P.dt / ts_cap<Real, NF>(P, Ts, m.S, m.rho) / m.rhos / P.ts.v_scale;
+            m.dTS += q * P.dt / ts_cap<Real, NF, M>(P, Ts, m.S, m.rho) / m.rhos / P.ts.v_scale;
         }
     }
+}
+
+// macro (LBM3D/_evolution.py:118-196) from gathered populations.
+// Tlag: lagged reaction temperature; TFold: TF.S of the previous step (porous
+// viscosity models); Sold: previous species S (mixture viscosity).
+template <class Real, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void macro(const DevParams<Real>& P, const CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m,
+                   Real Tlag, Real TFold, const Real* Sold) {
+    macro_flow<Real, D, NF, NX, M>(P, p.f, m, TFold, Sold);
+    macro_scalar<Real, D, NF, NX, M>(P, p, m, Tlag);
 }
 
 
 // ---------------------------------------------------------------------------
 // diffusivities (lattice units) of the scalar lattices
 // ---------------------------------------------------------------------------
-template <class Real, int NF, int NX>
+template <class Real, int NF, int NX, class M = ModelAny>
 LBM_DEV Real tf_D(const DevParams<Real>& P, const CellMac<Real, NF, NX>& m, Real kin) {
     const ThermP<Real>& t = P.tf;
-    if (t.diff_model == TDIFF_CONSTANT) return t.diff_lu;
-    if (t.diff_model == TDIFF_PRANDTL) return kin / t.Pr;
-    if (t.diff_model == TDIFF_DERIVED)
-        return tf_cond<Real, NF>(P, m.TF, m.S) / m.rho / tf_cap<Real, NF>(P, m.TF, m.S) * P.dtdx2;
+    if (LBM_MV(tf_diff, t.diff_model) == TDIFF_CONSTANT) return t.diff_lu;
+    if (LBM_MV(tf_diff, t.diff_model) == TDIFF_PRANDTL) return kin / t.Pr;
+    if (LBM_MV(tf_diff, t.diff_model) == TDIFF_DERIVED)
+        return tf_cond<Real, NF, M>(P, m.TF, m.S) / m.rho / tf_cap<Real, NF, M>(P, m.TF, m.S) * P.dtdx2;
     return (Real)0.0;
 }
-template <class Real, int NF, int NX>
+template <class Real, int NF, int NX, class M = ModelAny>
 LBM_DEV Real ts_D(const DevParams<Real>& P, const CellMac<Real, NF, NX>& m) {
     const ThermP<Real>& t = P.ts;
-    if (t.diff_model == TDIFF_CONSTANT) return t.diff_lu;
-    if (t.diff_model == TDIFF_DERIVED)
-        return ts_cond<Real, NF, NX>(P, m.TS, m.TF, m.S, m.rhos) / m.rhos / ts_cap<Real, NF>(P, m.TS, m.S, m.rho) * P.dtdx2;
+    if (LBM_MV(ts_diff, t.diff_model) == TDIFF_CONSTANT) return t.diff_lu;
+    if (LBM_MV(ts_diff, t.diff_model) == TDIFF_DERIVED)
+        return ts_cond<Real, NF, NX, M>(P, m.TS, m.TF, m.S, m.rhos) / m.rhos / ts_cap<Real, NF, M>(P, m.TS, m.S, m.rho) * P.dtdx2;
     return (Real)0.0;
 }
-template <class Real> LBM_DEV Real sp_D(const DevParams<Real>& P, int j, Real TFlu) {
+template <class Real, class M = ModelAny> LBM_DEV Real sp_D(const DevParams<Real>& P, int j, Real TFlu) {
     const SpecP<Real>& s = P.sp[j];
     Real diff = (Real)0.0;
-    if (s.diff_model == DIFF_CONSTANT) diff += s.diff_lu;
-    else if (s.diff_model == DIFF_SCHMIDT) diff += s.Sc * sp_visc(P, j, TFlu) * P.dtdx2;
+    if (LBM_MV(sp_diff, s.diff_model) == DIFF_CONSTANT) diff += s.diff_lu;
+    else if (LBM_MV(sp_diff, s.diff_model) == DIFF_SCHMIDT) diff += s.Sc * sp_visc<Real, M>(P, j, TFlu) * P.dtdx2;
     return diff;
 }
 
@@ -703,32 +748,32 @@ template <class Real> LBM_DEV Real sp_D(const DevParams<Real>& P, int j, Real TF
 //   scalar g* = (1 - 1/tau_s) g + w_k P_k (S/tau_s + dS),  P_k = geq_k / (w_k S)
 //          (equilibrium and source share P_k because both use the cell velocity)
 // ---------------------------------------------------------------------------
-template <class Real, int D, int NF, int NX>
-LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, bool pts) {
+template <class Real, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void collide_flow(const DevParams<Real>& P, Real* __restrict__ f, const CellMac<Real, NF, NX>& m) {
     using L = Lat<D>;
-    constexpr int Q = L::Q, QS = L::QS;
+    constexpr int Q = L::Q;
     const Real ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho, eps = m.eps;
-    const Real visc = visc_dyn<Real, NF>(P, m.TF, m.S);
-    const Real kin = kin_from_dyn(P, visc, rho);
+    const Real visc = visc_dyn<Real, NF, M>(P, m.TF, m.S);
+    const Real kin = kin_from_dyn<Real, M>(P, visc, rho);
     const Real tau = 3 * kin + (Real)0.5;
     const Real itau = 1 / tau;
     const Real omf = 1 - itau;
     // Guo forcing of the gray-LBM Darcy drag (darcy_force / forceTermGuo, :263-293)
     Real Fx = 0, Fy = 0, Fz = 0;
-    const bool guo = P.force_guo != 0;
-    if (guo && P.poro && m.sol != 0 && m.sol != 1) {
+    const bool guo = LBM_MV(force_guo, P.force_guo) != 0;
+    if (guo && LBM_MV(poro, P.poro) && m.sol != 0 && m.sol != 1) {
         const Real nu = D == 3 ? visc : kin;  // 3D dynamic, 2D kinematic viscosity
         const Real e = (Real)1.0 - m.sol;
         Real c = 0;
         bool any = true;
-        if (P.poro_model == PORO_SPHERICAL) {
+        if (LBM_MV(poro_model, P.poro_model) == PORO_SPHERICAL) {
             Real a = (Real)1.000001 - e;
             Real perm = e * e * e / (a * a);
             Real beta = (Real)1.75 / (Real)sqrt((Real)150 * e * perm);
             c = -nu / perm * e - beta * (Real)sqrt(ux * ux + uy * uy + uz * uz);
-        } else if (P.poro_model == PORO_DARCY || (D == 2 && P.poro_model == PORO_DARCY_HIGH)) {
+        } else if (LBM_MV(poro_model, P.poro_model) == PORO_DARCY || (D == 2 && LBM_MV(poro_model, P.poro_model) == PORO_DARCY_HIGH)) {
             c = -e * nu * m.cd_;
-        } else if (P.poro_model == PORO_DF) {
+        } else if (LBM_MV(poro_model, P.poro_model) == PORO_DF) {
             c = -e * nu * m.cd_ - e * m.cf_ * (Real)sqrt(ux * ux + uy * uy + uz * uz);
         } else {
             any = false;
@@ -739,7 +784,7 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
             Fz = c * uz;
         }
     }
-    const bool incomp = P.eos == EOS_INCOMP;
+    const bool incomp = LBM_MV(eos, P.eos) == EOS_INCOMP;
     const Real ee = (incomp && D == 2) ? eps : eps + (Real)1e-12;
     const Real iee = 1 / ee;
     const Real uv = ux * ux + uy * uy + uz * uz;
@@ -759,8 +804,17 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
         if (ey) { eu += ey * uy; eF += ey * Fy; }
         if (ez) { eu += ez * uz; eF += ez * Fz; }
         const Real poly = b0 + eu * (b1 + b2 * eu) + eF * (g3 + g9 * eu);
-        p.f[k] = omf * p.f[k] + (Real)L::w(k) * poly;
+        f[k] = omf * f[k] + (Real)L::w(k) * poly;
     }
+}
+
+template <class Real, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void collide_scalar(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, bool pts) {
+    using L = Lat<D>;
+    constexpr int QS = L::QS;
+    const Real ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
+    const Real uv = ux * ux + uy * uy + uz * uz;
+    const Real kin = kin_from_dyn<Real, M>(P, visc_dyn<Real, NF, M>(P, m.TF, m.S), rho);
     // scalar equilibria share P_k: 3D 1 + 4eu + 8eu^2 - 2u^2 (TF and species),
     // 2D TF 1 + 3eu + 4.5eu^2 - 1.5u^2, 2D species 1 + 3eu (linear)
     Real Pt[QS], Ps[QS];
@@ -775,8 +829,8 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
             Ps[k] = (Real)L::ws(k) * (1 + 3 * eu);
         }
     }
-    if (P.thermal) {
-        const Real itT = 1 / scalar_tau<D, Real>(tf_D<Real, NF, NX>(P, m, kin));
+    if (LBM_MV(thermal, P.thermal)) {
+        const Real itT = 1 / scalar_tau<D, Real>(tf_D<Real, NF, NX, M>(P, m, kin));
         const Real src = itT * m.TF + m.dTF;
         const Real om = 1 - itT;
 #pragma unroll
@@ -785,15 +839,15 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
     // FIX species: S += dS at direction 0; with a mixture solid conductivity the
     // TS relaxation time of directions >= 1 sees the updated densities.
     Real itS0 = 0, itS1 = 0;
-    if (P.thermal && pts) itS0 = 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX>(P, m));
-    if (P.chem) {
+    if (LBM_MV(thermal, P.thermal) && pts) itS0 = 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX, M>(P, m));
+    if (LBM_MV(chem, P.chem)) {
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (j < P.nfix) m.S[NF + j] += m.dS[NF + j];
+            if (j < (M::exact_species ? NX : P.nfix)) m.S[NF + j] += m.dS[NF + j];
     }
-    if (P.thermal && pts) {
-        itS1 = (P.chem && P.ts.diff_model == TDIFF_DERIVED && P.ts.cond_model == COND_MIXTURE)
-                   ? 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX>(P, m))
+    if (LBM_MV(thermal, P.thermal) && pts) {
+        itS1 = (LBM_MV(chem, P.chem) && LBM_MV(ts_diff, P.ts.diff_model) == TDIFF_DERIVED && LBM_MV(ts_cond, P.ts.cond_model) == COND_MIXTURE)
+                   ? 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX, M>(P, m))
                    : itS0;
 #pragma unroll
         for (int k = 0; k < QS; ++k) {
@@ -801,11 +855,11 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
             p.gS[k] = (1 - it) * p.gS[k] + (Real)L::ws(k) * (it * m.TS + m.dTS);
         }
     }
-    if (P.chem) {
+    if (LBM_MV(chem, P.chem)) {
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
-            if (j < P.nf) {
-                const Real it = 1 / scalar_tau<D, Real>(sp_D(P, j, m.TF));
+            if (j < (M::exact_species ? NF : P.nf)) {
+                const Real it = 1 / scalar_tau<D, Real>(sp_D<Real, M>(P, j, m.TF));
                 const Real src = it * m.S[j] + m.dS[j];
                 const Real om = 1 - it;
 #pragma unroll
@@ -815,11 +869,17 @@ LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, Cel
     }
 }
 
+template <class Real, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void collide(const DevParams<Real>& P, CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, bool pts) {
+    collide_flow<Real, D, NF, NX, M>(P, p.f, m);
+    collide_scalar<Real, D, NF, NX, M>(P, p, m, pts);
+}
+
 // TS of a fully solid cell: relax toward w*TS with no source (LBM3D/_evolution.py:98-104)
-template <class Real, int D, int NF, int NX>
+template <class Real, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void collide_emitter(const DevParams<Real>& P, Real* g, const CellMac<Real, NF, NX>& m) {
     using L = Lat<D>;
-    const Real it = 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX>(P, m));
+    const Real it = 1 / scalar_tau<D, Real>(ts_D<Real, NF, NX, M>(P, m));
 #pragma unroll
     for (int k = 0; k < L::QS; ++k) g[k] = g[k] - it * (g[k] - (Real)L::ws(k) * m.TS);
 }
@@ -952,65 +1012,65 @@ LBM_DEV void side_get(const Real* __restrict__ side, long long ns, int base, lon
     for (int k = 0; k < QN; ++k) out[k] = side[(long long)(base + k) * ns + si];
 }
 
-template <class Real, class Store, int D, int NF, int NX>
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void side_put_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long si,
                           const CellPops<Real, D, NF, NX>& p) {
     using L = Lat<D>;
     side_put<L::Q>(B.side, P.nshell, 0, si, p.f);
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         side_put<L::QS>(B.side, P.nshell, P.slot_tf, si, p.gT);
         side_put<L::QS>(B.side, P.nshell, P.slot_ts, si, p.gS);
     }
 #pragma unroll
     for (int j = 0; j < NF; ++j)
-        if (P.chem && j < P.nf) side_put<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) side_put<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
 }
-template <class Real, class Store, int D, int NF, int NX>
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void side_get_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long si,
                           CellPops<Real, D, NF, NX>& p) {
     using L = Lat<D>;
     side_get<L::Q>(B.side, P.nshell, 0, si, p.f);
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         side_get<L::QS>(B.side, P.nshell, P.slot_tf, si, p.gT);
         side_get<L::QS>(B.side, P.nshell, P.slot_ts, si, p.gS);
     }
 #pragma unroll
     for (int j = 0; j < NF; ++j)
-        if (P.chem && j < P.nf) side_get<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) side_get<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
 }
 
 // macroscopic fields
-template <class Real, class Store, int NF, int NX>
+template <class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void mac_put(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, const CellMac<Real, NF, NX>& m) {
     const long long N = P.N;
     B.rho[c] = m.rho;
     B.u[c] = m.ux;
     B.u[N + c] = m.uy;
     B.u[2 * N + c] = m.uz;
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         B.TF[c] = m.TF;
         B.TS[c] = m.TS;
         B.dTF[c] = m.dTF;
         B.dTS[c] = m.dTS;
         B.rhos[c] = m.rhos;
     }
-    if (P.chem) {
+    if (LBM_MV(chem, P.chem)) {
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (j < P.nf) {
+            if (j < (M::exact_species ? NF : P.nf)) {
                 B.S[(long long)j * N + c] = m.S[j];
                 B.dS[(long long)j * N + c] = m.dS[j];
             }
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (j < P.nfix) {
-                B.S[(long long)(P.nf + j) * N + c] = m.S[NF + j];
-                B.dS[(long long)(P.nf + j) * N + c] = m.dS[NF + j];
+            if (j < (M::exact_species ? NX : P.nfix)) {
+                B.S[(long long)((M::exact_species ? NF : P.nf) + j) * N + c] = m.S[NF + j];
+                B.dS[(long long)((M::exact_species ? NF : P.nf) + j) * N + c] = m.dS[NF + j];
             }
-        if (P.poro) B.rhos[c] = m.rhos;
+        if (LBM_MV(poro, P.poro)) B.rhos[c] = m.rhos;
     }
 }
-template <class Real, class Store, int NF, int NX>
+template <class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void mac_get(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, CellMac<Real, NF, NX>& m) {
     const long long N = P.N;
     m.rho = B.rho[c];
@@ -1019,7 +1079,7 @@ LBM_DEV void mac_get(const DevParams<Real>& P, const Bufs<Real, Store>& B, long 
     m.uz = B.u[2 * N + c];
     m.TF = m.TS = m.dTF = m.dTS = (Real)0;
     m.rhos = (Real)0;
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         m.TF = B.TF[c];
         m.TS = B.TS[c];
         m.dTF = B.dTF[c];
@@ -1031,141 +1091,250 @@ LBM_DEV void mac_get(const DevParams<Real>& P, const Bufs<Real, Store>& B, long 
         m.S[j] = (Real)0;
         m.dS[j] = (Real)0;
     }
-    if (P.chem) {
+    if (LBM_MV(chem, P.chem)) {
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (j < P.nf) {
+            if (j < (M::exact_species ? NF : P.nf)) {
                 m.S[j] = B.S[(long long)j * N + c];
                 m.dS[j] = B.dS[(long long)j * N + c];
             }
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (j < P.nfix) {
-                m.S[NF + j] = B.S[(long long)(P.nf + j) * N + c];
-                m.dS[NF + j] = B.dS[(long long)(P.nf + j) * N + c];
+            if (j < (M::exact_species ? NX : P.nfix)) {
+                m.S[NF + j] = B.S[(long long)((M::exact_species ? NF : P.nf) + j) * N + c];
+                m.dS[NF + j] = B.dS[(long long)((M::exact_species ? NF : P.nf) + j) * N + c];
             }
-        if (P.poro) m.rhos = B.rhos[c];
+        if (LBM_MV(poro, P.poro)) m.rhos = B.rhos[c];
     }
 }
 
-template <class Real, class Store, int NF, int NX>
+template <class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void load_static(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, uint32_t mask,
                          CellMac<Real, NF, NX>& m) {
     m.sol = (mask & SOLIDNZ_BIT) ? B.solid[c] : (Real)0;
     m.eps = 1 - m.sol;
     m.cd_ = m.cf_ = m.ha_ = m.rad_ = (Real)0;
-    if (P.poro && m.sol != 0) {
-        if (P.poro_model == PORO_DARCY || P.poro_model == PORO_DF || P.poro_model == PORO_DARCY_HIGH) m.cd_ = B.cd[c];
-        if (P.poro_model == PORO_DF) m.cf_ = B.cf[c];
+    if (LBM_MV(poro, P.poro) && m.sol != 0) {
+        if (LBM_MV(poro_model, P.poro_model) == PORO_DARCY || LBM_MV(poro_model, P.poro_model) == PORO_DF || LBM_MV(poro_model, P.poro_model) == PORO_DARCY_HIGH) m.cd_ = B.cd[c];
+        if (LBM_MV(poro_model, P.poro_model) == PORO_DF) m.cf_ = B.cf[c];
     }
-    if (P.thermal && m.sol > 0) m.ha_ = B.ha[c];
-    if (P.thermal && P.radiation) m.rad_ = B.rad[c];
+    if (LBM_MV(thermal, P.thermal) && m.sol > 0) m.ha_ = B.ha[c];
+    if (LBM_MV(thermal, P.thermal) && LBM_MV(radiation, P.radiation)) m.rad_ = B.rad[c];
 }
 
-template <class Real, class Store, int NF, int NX>
+template <class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void lag_put(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, const CellMac<Real, NF, NX>& m) {
-    if (P.chem && P.thermal) B.lag[c] = m.sol > 0 ? m.TS : m.TF;
-    if (P.lag_tf) B.lagTF[c] = m.TF;
-    if (P.lag_sp) {
+    if (LBM_MV(chem, P.chem) && LBM_MV(thermal, P.thermal)) B.lag[c] = m.sol > 0 ? m.TS : m.TF;
+    if (LBM_MV(lag_tf, P.lag_tf)) B.lagTF[c] = m.TF;
+    if (LBM_MV(lag_sp, P.lag_sp)) {
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (j < P.nf) B.lagS[(long long)j * P.N + c] = m.S[j];
+            if (j < (M::exact_species ? NF : P.nf)) B.lagS[(long long)j * P.N + c] = m.S[j];
     }
 }
 
-template <int D, int PAR, class Real, class Store, int NF, int NX>
+template <int D, int PAR, class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void scatter_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, const Geo& g, uint32_t mask,
                          const CellPops<Real, D, NF, NX>& p, bool pts) {
     using L = Lat<D>;
     scatter<D, L::Q, PAR>(B.pops, P.N, 0, g, mask, 0, p.f);
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_tf, g, mask, 0, p.gT);
         if (pts) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_ts, g, mask, TS_BIT, p.gS);
         else store_fixed<L::QS>(B.pops, P.N, P.slot_ts, g.c, p.gS);
     }
 #pragma unroll
     for (int j = 0; j < NF; ++j)
-        if (P.chem && j < P.nf) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_sp + j * L::QS, g, mask, 0, p.gP[j]);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_sp + j * L::QS, g, mask, 0, p.gP[j]);
 }
 
 // ---------------------------------------------------------------------------
 // NEE of a fast-axis face cell from its inner neighbour's state (LBM3D/_boundary.py:
 // 260-282 for z-faces, LBM2D/_boundary.py:80-101 for y-faces; scalar part
-// LBM3D/_scalarField.py:123-141).  p/m hold the face cell's gathered populations and
-// pre-BC moments on entry and its post-BC populations and moments on return; n*
-// are the neighbour's populations, moments and solid fraction.
+// LBM3D/_scalarField.py:123-141).  The neighbour is lane `src` of the same wavefront;
+// every lane executes the shuffles, only lanes with `isface` change p/m (gathered
+// populations and pre-BC moments on entry, post-BC values on return).
 // ---------------------------------------------------------------------------
-template <class Real, class Store, int D, int NF, int NX>
-LBM_DEV void nee_inwave(const DevParams<Real>& P, const Bufs<Real, Store>& B, int face, long long fc,
-                        CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m, const CellPops<Real, D, NF, NX>& pn,
-                        Real nrho, Real nux, Real nuy, Real nuz, Real nTF, Real nTS, const Real* nS, Real nsol) {
+template <class T> LBM_DEV T lane_fetch(T v, int src) { return __shfl(v, src, 64); }
+
+// flow part of the in-wave NEE; returns the neighbour velocity for the scalar part
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void nee_inwave_flow(const DevParams<Real>& P, const Bufs<Real, Store>& B, bool isface, int face, long long fc,
+                             int src, Real* __restrict__ f, CellMac<Real, NF, NX>& m, Real& nux, Real& nuy, Real& nuz) {
     using L = Lat<D>;
-    constexpr int Q = L::Q, QS = L::QS;
+    const Real nrho = lane_fetch(m.rho, src), nsol = lane_fetch(m.sol, src);
+    nux = lane_fetch(m.ux, src);
+    nuy = lane_fetch(m.uy, src);
+    nuz = lane_fetch(m.uz, src);
     Real ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
-    const int bv = P.bc_v[face], br = P.bc_rho[face];
-    if (bv == BC_FIXED) {
-        const Real* vp = B.vprof + B.prof_off[face] * 3 + fc * 3;
-        ux = vp[0];
-        uy = vp[1];
-        uz = vp[2];
-    } else if (bv == BC_ZEROGRAD) {
-        ux = nux;
-        uy = nuy;
-        uz = nuz;
+    if (isface) {
+        const int bv = P.bc_v[face], br = P.bc_rho[face];
+        if (bv == BC_FIXED) {
+            const Real* vp = B.vprof + B.prof_off[face] * 3 + fc * 3;
+            ux = vp[0];
+            uy = vp[1];
+            uz = vp[2];
+        } else if (bv == BC_ZEROGRAD) {
+            ux = nux;
+            uy = nuy;
+            uz = nuz;
+        }
+        if (br == BC_FIXED) rho = P.rho_bc[face];
+        else if (br == BC_ZEROGRAD && !(D == 3 && face == 4)) rho = nrho;
     }
-    if (br == BC_FIXED) rho = P.rho_bc[face];
-    else if (br == BC_ZEROGRAD && !(D == 3 && face == 4)) rho = nrho;
     const Real eb = 1 - m.sol, enb = 1 - nsol;
+    const int eos = LBM_MV(eos, P.eos);
 #pragma unroll
-    for (int k = 0; k < Q; ++k)
-        p.f[k] = feq<D, Real>(P.eos, k, rho, ux, uy, uz, eb) + (pn.f[k] - feq<D, Real>(P.eos, k, nrho, nux, nuy, nuz, enb));
-    if (P.chem) {
+    for (int k = 0; k < L::Q; ++k) {
+        const Real fn = lane_fetch(f[k], src);
+        if (isface) f[k] = feq<D, Real>(eos, k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(eos, k, nrho, nux, nuy, nuz, enb));
+    }
+    if (isface) {
+        m.rho = rho;
+        m.ux = ux;
+        m.uy = uy;
+        m.uz = uz;
+    }
+}
+// scalar part of the in-wave NEE (face velocity already in m)
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void nee_inwave_scalar(const DevParams<Real>& P, bool isface, int face, int src, CellPops<Real, D, NF, NX>& p,
+                               CellMac<Real, NF, NX>& m, Real nux, Real nuy, Real nuz) {
+    using L = Lat<D>;
+    constexpr int QS = L::QS;
+    const Real ux = m.ux, uy = m.uy, uz = m.uz;
+    if (LBM_MV(chem, P.chem)) {
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
-            if (j < P.nf) {
+            if (j < (M::exact_species ? NF : P.nf)) {
                 const SpecP<Real>& sp = P.sp[j];
+                const Real nS = lane_fetch(m.S[j], src);
                 Real Sb = m.S[j];
                 if (sp.bc_type[face] == BC_FIXED) Sb = sp.bc_value[face];
-                if (sp.bc_type[face] == BC_ZEROGRAD) Sb = nS[j];
+                if (sp.bc_type[face] == BC_ZEROGRAD) Sb = nS;
 #pragma unroll
-                for (int k = 0; k < QS; ++k)
-                    p.gP[j][k] = geq_sp<D, Real>(k, Sb, ux, uy, uz) + (pn.gP[j][k] - geq_sp<D, Real>(k, nS[j], nux, nuy, nuz));
-                m.S[j] = Sb;
+                for (int k = 0; k < QS; ++k) {
+                    const Real gn = lane_fetch(p.gP[j][k], src);
+                    if (isface) p.gP[j][k] = geq_sp<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_sp<D, Real>(k, nS, nux, nuy, nuz));
+                }
+                if (isface) m.S[j] = Sb;
             }
         }
     }
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
+        const Real nTF = lane_fetch(m.TF, src), nTS = lane_fetch(m.TS, src);
         Real Tb = m.TF;
         if (P.tf.bc_type[face] == BC_FIXED) Tb = P.tf.bc_value[face];
         if (P.tf.bc_type[face] == BC_ZEROGRAD) Tb = nTF;
 #pragma unroll
-        for (int k = 0; k < QS; ++k)
-            p.gT[k] = geq_tf<D, Real>(k, Tb, ux, uy, uz) + (pn.gT[k] - geq_tf<D, Real>(k, nTF, nux, nuy, nuz));
-        m.TF = Tb;
+        for (int k = 0; k < QS; ++k) {
+            const Real gn = lane_fetch(p.gT[k], src);
+            if (isface) p.gT[k] = geq_tf<D, Real>(k, Tb, ux, uy, uz) + (gn - geq_tf<D, Real>(k, nTF, nux, nuy, nuz));
+        }
         Real Sb = m.TS;
         if (P.ts.bc_type[face] == BC_FIXED) Sb = P.ts.bc_value[face];
         if (P.ts.bc_type[face] == BC_ZEROGRAD) Sb = nTS;
 #pragma unroll
         for (int k = 0; k < QS; ++k) {
+            const Real gn = lane_fetch(p.gS[k], src);
             const Real w = (Real)L::ws(k);
-            p.gS[k] = w * Sb + (pn.gS[k] - w * nTS);
+            if (isface) p.gS[k] = w * Sb + (gn - w * nTS);
         }
-        m.TS = Sb;
+        if (isface) {
+            m.TF = Tb;
+            m.TS = Sb;
+        }
     }
-    m.rho = rho;
-    m.ux = ux;
-    m.uy = uy;
-    m.uz = uz;
 }
 
-template <class T> LBM_DEV T lane_fetch(T v, int src) { return __shfl(v, src, 64); }
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
+LBM_DEV void nee_inwave(const DevParams<Real>& P, const Bufs<Real, Store>& B, bool isface, int face, long long fc,
+                        int src, CellPops<Real, D, NF, NX>& p, CellMac<Real, NF, NX>& m) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q, QS = L::QS;
+    const Real nrho = lane_fetch(m.rho, src), nux = lane_fetch(m.ux, src), nuy = lane_fetch(m.uy, src),
+               nuz = lane_fetch(m.uz, src), nsol = lane_fetch(m.sol, src);
+    Real ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
+    if (isface) {
+        const int bv = P.bc_v[face], br = P.bc_rho[face];
+        if (bv == BC_FIXED) {
+            const Real* vp = B.vprof + B.prof_off[face] * 3 + fc * 3;
+            ux = vp[0];
+            uy = vp[1];
+            uz = vp[2];
+        } else if (bv == BC_ZEROGRAD) {
+            ux = nux;
+            uy = nuy;
+            uz = nuz;
+        }
+        if (br == BC_FIXED) rho = P.rho_bc[face];
+        else if (br == BC_ZEROGRAD && !(D == 3 && face == 4)) rho = nrho;
+    }
+    const Real eb = 1 - m.sol, enb = 1 - nsol;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const Real fn = lane_fetch(p.f[k], src);
+        if (isface)
+            p.f[k] = feq<D, Real>(LBM_MV(eos, P.eos), k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(LBM_MV(eos, P.eos), k, nrho, nux, nuy, nuz, enb));
+    }
+    if (LBM_MV(chem, P.chem)) {
+#pragma unroll
+        for (int j = 0; j < NF; ++j) {
+            if (j < (M::exact_species ? NF : P.nf)) {
+                const SpecP<Real>& sp = P.sp[j];
+                const Real nS = lane_fetch(m.S[j], src);
+                Real Sb = m.S[j];
+                if (sp.bc_type[face] == BC_FIXED) Sb = sp.bc_value[face];
+                if (sp.bc_type[face] == BC_ZEROGRAD) Sb = nS;
+#pragma unroll
+                for (int k = 0; k < QS; ++k) {
+                    const Real gn = lane_fetch(p.gP[j][k], src);
+                    if (isface) p.gP[j][k] = geq_sp<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_sp<D, Real>(k, nS, nux, nuy, nuz));
+                }
+                if (isface) m.S[j] = Sb;
+            }
+        }
+    }
+    if (LBM_MV(thermal, P.thermal)) {
+        const Real nTF = lane_fetch(m.TF, src), nTS = lane_fetch(m.TS, src);
+        Real Tb = m.TF;
+        if (P.tf.bc_type[face] == BC_FIXED) Tb = P.tf.bc_value[face];
+        if (P.tf.bc_type[face] == BC_ZEROGRAD) Tb = nTF;
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            const Real gn = lane_fetch(p.gT[k], src);
+            if (isface) p.gT[k] = geq_tf<D, Real>(k, Tb, ux, uy, uz) + (gn - geq_tf<D, Real>(k, nTF, nux, nuy, nuz));
+        }
+        Real Sb = m.TS;
+        if (P.ts.bc_type[face] == BC_FIXED) Sb = P.ts.bc_value[face];
+        if (P.ts.bc_type[face] == BC_ZEROGRAD) Sb = nTS;
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            const Real gn = lane_fetch(p.gS[k], src);
+            const Real w = (Real)L::ws(k);
+            if (isface) p.gS[k] = w * Sb + (gn - w * nTS);
+        }
+        if (isface) {
+            m.TF = Tb;
+            m.TS = Sb;
+        }
+    }
+    if (isface) {
+        m.rho = rho;
+        m.ux = ux;
+        m.uy = uy;
+        m.uz = uz;
+    }
+}
+
+
 
 // ---------------------------------------------------------------------------
 // kernel bodies
 // ---------------------------------------------------------------------------
 // Bulk: gather (streaming of collision n) -> macro n -> [shell: record] ->
 // [interior: collision n+1 + scatter].
-template <class Real, class Store, int D, int NF, int NX, int PAR>
+template <class Real, class Store, int D, int NF, int NX, int PAR, class M = ModelAny>
 LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, int write_macro) {
     using L = Lat<D>;
     constexpr int QS = L::QS;
@@ -1186,34 +1355,123 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
         // fully solid cell: flow, TF and species are frozen; TS relaxes from its own populations
         if (shell) {
             load_fixed<L::Q>(B.pops, N, 0, c, p.f);
-            if (P.thermal) {
+            if (LBM_MV(thermal, P.thermal)) {
                 load_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
                 gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
             }
 #pragma unroll
             for (int j = 0; j < NF; ++j)
-                if (P.chem && j < P.nf) load_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
-            side_put_all<Real, Store, D, NF, NX>(P, B, si, p);
+                if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) load_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+            side_put_all<Real, Store, D, NF, NX, M>(P, B, si, p);
         }
-        if (face || !P.thermal) return;
-        mac_get<Real, Store, NF, NX>(P, B, c, m);
-        load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+        if (face || !LBM_MV(thermal, P.thermal)) return;
+        mac_get<Real, Store, NF, NX, M>(P, B, c, m);
+        load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (P.chem && j < P.nfix) m.S[NF + j] = B.fixS[(long long)j * N + c];
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) m.S[NF + j] = B.fixS[(long long)j * N + c];
         Real ge[QS];
 #pragma unroll
         for (int k = 0; k < QS; ++k) ge[k] = B.emit[(long long)k * N + c];
-        collide_emitter<Real, D, NF, NX>(P, ge, m);
+        collide_emitter<Real, D, NF, NX, M>(P, ge, m);
 #pragma unroll
         for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = ge[k];
         scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, ge);
         return;
     }
 
-    load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+    if constexpr (M::visc == VISC_CONSTANT || M::visc == VISC_NONE) {
+        // the flow collision does not read the scalar state when the viscosity is a
+        // constant: finish the flow lattice first, then the scalar lattices, so only
+        // one group of populations is live at a time
+        load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
+        Real f[L::Q];
+        gather<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, f);
+        {
+            Real Sdummy[NF > 0 ? NF : 1];
+#pragma unroll
+            for (int j = 0; j < (NF > 0 ? NF : 1); ++j) Sdummy[j] = (Real)0;
+            macro_flow<Real, D, NF, NX, M>(P, f, m, (Real)0, Sdummy);
+        }
+        const int fpos = D == 3 ? g.z : g.y;
+        const int nfast = D == 3 ? P.nz : P.ny;
+        const bool zface = zfast && (fpos == 0 || fpos == nfast - 1);
+        const bool anyz = __ballot(zface) != 0;
+        const int src = (int)(threadIdx.x & 63) + (zface ? (fpos == 0 ? 1 : -1) : 0);
+        const int zfid = D == 3 ? (fpos == 0 ? 4 : 5) : (fpos == 0 ? 2 : 3);
+        Real nux = 0, nuy = 0, nuz = 0;
+        if (anyz) {
+            const long long fc = D == 3 ? (long long)g.x * P.ny + g.y : (long long)g.x;
+            nee_inwave_flow<Real, Store, D, NF, NX, M>(P, B, zface, zfid, fc, src, f, m, nux, nuy, nuz);
+        }
+        if (shell) side_put<L::Q>(B.side, P.nshell, 0, si, f);
+        if (!face) {
+            collide_flow<Real, D, NF, NX, M>(P, f, m);
+            scatter<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, f);
+        }
+        // ---- scalar lattices ----
+        if (LBM_MV(thermal, P.thermal)) {
+            gather<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
+            if (pts) gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+            else if (shell || write_macro || zfast) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+            else {
+#pragma unroll
+                for (int k = 0; k < QS; ++k) p.gS[k] = (Real)0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf))
+                gather<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+#pragma unroll
+        for (int j = 0; j < NF + NX; ++j) {
+            m.S[j] = (Real)0;
+            m.dS[j] = (Real)0;
+        }
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) m.S[NF + j] = B.fixS[(long long)j * N + c];
+        m.rhos = (Real)0;
+        if (LBM_MV(thermal, P.thermal) && !(LBM_MV(chem, P.chem) && LBM_MV(poro, P.poro)) &&
+            (m.sol > 0 || LBM_MV(radiation, P.radiation) || shell || write_macro))
+            m.rhos = B.rhos[c];
+        const Real Tlag = (LBM_MV(chem, P.chem) && LBM_MV(thermal, P.thermal)) ? B.lag[c] : (Real)0;
+        macro_scalar<Real, D, NF, NX, M>(P, p, m, Tlag);
+        if (anyz) nee_inwave_scalar<Real, Store, D, NF, NX, M>(P, zface, zfid, src, p, m, nux, nuy, nuz);
+        if (shell) {
+            if (LBM_MV(thermal, P.thermal)) {
+                side_put<QS>(B.side, P.nshell, P.slot_tf, si, p.gT);
+                side_put<QS>(B.side, P.nshell, P.slot_ts, si, p.gS);
+            }
+#pragma unroll
+            for (int j = 0; j < NF; ++j)
+                if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf))
+                    side_put<QS>(B.side, P.nshell, P.slot_sp + j * QS, si, p.gP[j]);
+            mac_put<Real, Store, NF, NX, M>(P, B, c, m);
+            if (face) return;
+        } else if (write_macro) {
+            mac_put<Real, Store, NF, NX, M>(P, B, c, m);
+        }
+        lag_put<Real, Store, NF, NX, M>(P, B, c, m);
+        collide_scalar<Real, D, NF, NX, M>(P, p, m, pts);
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
+        if (LBM_MV(thermal, P.thermal)) {
+            scatter<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
+            if (pts) scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+            else store_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+        }
+#pragma unroll
+        for (int j = 0; j < NF; ++j)
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf))
+                scatter<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+        return;
+    }
+
+    load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
     gather<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, p.f);
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         gather<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
         if (pts) gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
         else if (shell || write_macro || zfast) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
@@ -1224,7 +1482,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     }
 #pragma unroll
     for (int j = 0; j < NF; ++j)
-        if (P.chem && j < P.nf) gather<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) gather<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
 #pragma unroll
     for (int j = 0; j < NF + NX; ++j) {
         m.S[j] = (Real)0;
@@ -1232,15 +1490,15 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     }
 #pragma unroll
     for (int j = 0; j < NX; ++j)
-        if (P.chem && j < P.nfix) m.S[NF + j] = B.fixS[(long long)j * N + c];
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) m.S[NF + j] = B.fixS[(long long)j * N + c];
     m.rhos = (Real)0;
-    if (P.thermal && !(P.chem && P.poro) && (m.sol > 0 || P.radiation || shell || write_macro)) m.rhos = B.rhos[c];
-    const Real Tlag = (P.chem && P.thermal) ? B.lag[c] : (Real)0;
-    const Real TFold = P.lag_tf ? B.lagTF[c] : Tlag;
+    if (LBM_MV(thermal, P.thermal) && !(LBM_MV(chem, P.chem) && LBM_MV(poro, P.poro)) && (m.sol > 0 || LBM_MV(radiation, P.radiation) || shell || write_macro)) m.rhos = B.rhos[c];
+    const Real Tlag = (LBM_MV(chem, P.chem) && LBM_MV(thermal, P.thermal)) ? B.lag[c] : (Real)0;
+    const Real TFold = LBM_MV(lag_tf, P.lag_tf) ? B.lagTF[c] : Tlag;
     Real Sold[NF > 0 ? NF : 1];
 #pragma unroll
-    for (int j = 0; j < NF; ++j) Sold[j] = (P.lag_sp && j < P.nf) ? B.lagS[(long long)j * N + c] : (Real)0;
-    macro<Real, D, NF, NX>(P, p, m, Tlag, TFold, Sold);
+    for (int j = 0; j < NF; ++j) Sold[j] = (LBM_MV(lag_sp, P.lag_sp) && j < (M::exact_species ? NF : P.nf)) ? B.lagS[(long long)j * N + c] : (Real)0;
+    macro<Real, D, NF, NX, M>(P, p, m, Tlag, TFold, Sold);
 
     // fast-axis face cells of deep rows: NEE from the neighbouring lane, then proceed
     // like an interior cell (collision from the post-BC state)
@@ -1250,57 +1508,30 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     if (__ballot(zface) != 0) {
         const int lane = (int)(threadIdx.x & 63);
         const int src = lane + (zface ? (fpos == 0 ? 1 : -1) : 0);
-        CellPops<Real, D, NF, NX> pn;
-#pragma unroll
-        for (int k = 0; k < L::Q; ++k) pn.f[k] = lane_fetch(p.f[k], src);
-        Real nTF = 0, nTS = 0;
-        if (P.thermal) {
-#pragma unroll
-            for (int k = 0; k < QS; ++k) {
-                pn.gT[k] = lane_fetch(p.gT[k], src);
-                pn.gS[k] = lane_fetch(p.gS[k], src);
-            }
-            nTF = lane_fetch(m.TF, src);
-            nTS = lane_fetch(m.TS, src);
-        }
-        Real nS[NF > 0 ? NF : 1];
-#pragma unroll
-        for (int j = 0; j < NF; ++j) {
-            nS[j] = (Real)0;
-            if (P.chem && j < P.nf) {
-#pragma unroll
-                for (int k = 0; k < QS; ++k) pn.gP[j][k] = lane_fetch(p.gP[j][k], src);
-                nS[j] = lane_fetch(m.S[j], src);
-            }
-        }
-        const Real nrho = lane_fetch(m.rho, src), nux = lane_fetch(m.ux, src), nuy = lane_fetch(m.uy, src),
-                   nuz = lane_fetch(m.uz, src), nsol = lane_fetch(m.sol, src);
-        if (zface) {
-            const int face = D == 3 ? (fpos == 0 ? 4 : 5) : (fpos == 0 ? 2 : 3);
-            const long long fc = D == 3 ? (long long)g.x * P.ny + g.y : (long long)g.x;
-            nee_inwave<Real, Store, D, NF, NX>(P, B, face, fc, p, m, pn, nrho, nux, nuy, nuz, nTF, nTS, nS, nsol);
-        }
+        const int face = D == 3 ? (fpos == 0 ? 4 : 5) : (fpos == 0 ? 2 : 3);
+        const long long fc = D == 3 ? (long long)g.x * P.ny + g.y : (long long)g.x;
+        nee_inwave<Real, Store, D, NF, NX, M>(P, B, zface, face, fc, src, p, m);
     }
 
     if (shell) {
-        side_put_all<Real, Store, D, NF, NX>(P, B, si, p);
-        mac_put<Real, Store, NF, NX>(P, B, c, m);
+        side_put_all<Real, Store, D, NF, NX, M>(P, B, si, p);
+        mac_put<Real, Store, NF, NX, M>(P, B, c, m);
         if (face) return;
     } else if (write_macro) {
-        mac_put<Real, Store, NF, NX>(P, B, c, m);
+        mac_put<Real, Store, NF, NX, M>(P, B, c, m);
     }
-    lag_put<Real, Store, NF, NX>(P, B, c, m);
-    collide<Real, D, NF, NX>(P, p, m, pts);
+    lag_put<Real, Store, NF, NX, M>(P, B, c, m);
+    collide<Real, D, NF, NX, M>(P, p, m, pts);
 #pragma unroll
     for (int j = 0; j < NX; ++j)
-        if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
-    scatter_all<D, PAR, Real, Store, NF, NX>(P, B, g, mask, p, pts);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
+    scatter_all<D, PAR, Real, Store, NF, NX, M>(P, B, g, mask, p, pts);
 }
 
 // NEE of one face cell b with inner neighbour nb (LBM3D/_boundary.py:211-282,
 // LBM3D/_scalarField.py:83-141).  Reads and writes the side buffer and the
 // macroscopic fields.
-template <class Real, class Store, int D, int NF, int NX>
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int face, int bx, int by, int bz, int nx_,
                       int ny_, int nz_, long long fc) {
     using L = Lat<D>;
@@ -1337,12 +1568,12 @@ LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int 
     for (int k = 0; k < Q; ++k) {
         const Real fn = B.side[(long long)k * P.nshell + snb];
         B.side[(long long)k * P.nshell + sb] =
-            feq<D, Real>(P.eos, k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(P.eos, k, nrho, nux, nuy, nuz, enb));
+            feq<D, Real>(LBM_MV(eos, P.eos), k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(LBM_MV(eos, P.eos), k, nrho, nux, nuy, nuz, enb));
     }
-    if (P.chem) {
+    if (LBM_MV(chem, P.chem)) {
 #pragma unroll
         for (int j = 0; j < NF; ++j) {
-            if (j < P.nf) {
+            if (j < (M::exact_species ? NF : P.nf)) {
                 const SpecP<Real>& s = P.sp[j];
                 Real Sb = B.S[(long long)j * N + b];
                 const Real Sn = B.S[(long long)j * N + nb];
@@ -1359,7 +1590,7 @@ LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int 
             }
         }
     }
-    if (P.thermal) {
+    if (LBM_MV(thermal, P.thermal)) {
         {
             Real Sb = B.TF[b];
             const Real Sn = B.TF[nb];
@@ -1390,7 +1621,7 @@ LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int 
 }
 
 // collision n+1 of a face cell from its post-NEE state
-template <class Real, class Store, int D, int NF, int NX, int PAR>
+template <class Real, class Store, int D, int NF, int NX, int PAR, class M = ModelAny>
 LBM_DEV void face_collide_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long fi) {
     using L = Lat<D>;
     constexpr int QS = L::QS;
@@ -1404,37 +1635,37 @@ LBM_DEV void face_collide_cell(const DevParams<Real>& P, const Bufs<Real, Store>
     const long long si = band_index<D>(g.x, g.y, g.z, P.nx, P.ny, P.nz, 2);
     CellPops<Real, D, NF, NX> p;
     CellMac<Real, NF, NX> m;
-    side_get_all<Real, Store, D, NF, NX>(P, B, si, p);
-    mac_get<Real, Store, NF, NX>(P, B, c, m);
-    load_static<Real, Store, NF, NX>(P, B, c, mask, m);
+    side_get_all<Real, Store, D, NF, NX, M>(P, B, si, p);
+    mac_get<Real, Store, NF, NX, M>(P, B, c, m);
+    load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
 #pragma unroll
     for (int j = 0; j < NX; ++j)
-        if (P.chem && j < P.nfix) m.S[NF + j] = B.fixS[(long long)j * N + c];
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) m.S[NF + j] = B.fixS[(long long)j * N + c];
     if (!pflow) {
         store_fixed<L::Q>(B.pops, N, 0, c, p.f);
-        if (P.thermal) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+        if (LBM_MV(thermal, P.thermal)) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (P.chem && j < P.nf) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
-        if (P.thermal) {
-            collide_emitter<Real, D, NF, NX>(P, p.gS, m);
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+        if (LBM_MV(thermal, P.thermal)) {
+            collide_emitter<Real, D, NF, NX, M>(P, p.gS, m);
 #pragma unroll
             for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = p.gS[k];
             scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
         }
         return;
     }
-    lag_put<Real, Store, NF, NX>(P, B, c, m);
-    collide<Real, D, NF, NX>(P, p, m, pts);
+    lag_put<Real, Store, NF, NX, M>(P, B, c, m);
+    collide<Real, D, NF, NX, M>(P, p, m, pts);
 #pragma unroll
     for (int j = 0; j < NX; ++j)
-        if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
-    scatter_all<D, PAR, Real, Store, NF, NX>(P, B, g, mask, p, pts);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
+    scatter_all<D, PAR, Real, Store, NF, NX, M>(P, B, g, mask, p, pts);
 }
 
 // Equilibrium initialisation (LBM3D/_initialize.py:45-77) followed by the
 // collision of step 1, written with the even-step layout.
-template <class Real, class Store, int D, int NF, int NX>
+template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, int inflow_x2, Real v0) {
     using L = Lat<D>;
     constexpr int Q = L::Q, QS = L::QS;
@@ -1446,18 +1677,18 @@ LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     const bool pts = ((mask >> TS_BIT) & 1u) != 0;
     CellPops<Real, D, NF, NX> p;
     CellMac<Real, NF, NX> m;
-    mac_get<Real, Store, NF, NX>(P, B, c, m);
-    load_static<Real, Store, NF, NX>(P, B, c, mask, m);
-    if (P.chem && P.poro) {
+    mac_get<Real, Store, NF, NX, M>(P, B, c, m);
+    load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
+    if (LBM_MV(chem, P.chem) && LBM_MV(poro, P.poro)) {
         Real rhos = (Real)0.0;
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (j < P.nfix) rhos += m.S[NF + j];
+            if (j < (M::exact_species ? NX : P.nfix)) rhos += m.S[NF + j];
         m.rhos = rhos;
         B.rhos[c] = rhos;
     }
 #pragma unroll
-    for (int k = 0; k < Q; ++k) p.f[k] = feq<D, Real>(P.eos, k, m.rho, m.ux, m.uy, m.uz, m.eps);
+    for (int k = 0; k < Q; ++k) p.f[k] = feq<D, Real>(LBM_MV(eos, P.eos), k, m.rho, m.ux, m.uy, m.uz, m.eps);
 #pragma unroll
     for (int k = 0; k < QS; ++k) {
         p.gT[k] = geq_tf<D, Real>(k, m.TF, m.ux, m.uy, m.uz);
@@ -1476,27 +1707,27 @@ LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     }
     if (!pflow) {
         store_fixed<Q>(B.pops, N, 0, c, p.f);
-        if (P.thermal) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+        if (LBM_MV(thermal, P.thermal)) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (P.chem && j < P.nf) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
 #pragma unroll
         for (int j = 0; j < NX; ++j)
-            if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
-        if (P.thermal) {
-            collide_emitter<Real, D, NF, NX>(P, p.gS, m);
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
+        if (LBM_MV(thermal, P.thermal)) {
+            collide_emitter<Real, D, NF, NX, M>(P, p.gS, m);
 #pragma unroll
             for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = p.gS[k];
             scatter<D, QS, 0>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
         }
         return;
     }
-    lag_put<Real, Store, NF, NX>(P, B, c, m);
-    collide<Real, D, NF, NX>(P, p, m, pts);
+    lag_put<Real, Store, NF, NX, M>(P, B, c, m);
+    collide<Real, D, NF, NX, M>(P, p, m, pts);
 #pragma unroll
     for (int j = 0; j < NX; ++j)
-        if (P.chem && j < P.nfix) B.fixS[(long long)j * N + c] = m.S[NF + j];
-    scatter_all<D, 0, Real, Store, NF, NX>(P, B, g, mask, p, pts);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
+    scatter_all<D, 0, Real, Store, NF, NX, M>(P, B, g, mask, p, pts);
 }
 
 // per-cell flags (solid is static after init)

@@ -49,13 +49,22 @@ __global__ __launch_bounds__(BLOCK) void k_init(const DevParams<Real>* __restric
     init_cell<Real, Store, D, NF, NX>(P, B, c, inflow, v0);
 }
 
-template <class Real, class Store, int D, int NF, int NX, int PAR>
-__global__ __launch_bounds__(BLOCK) void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B,
-                                                int write_macro) {
+// The specialised kernels carry one population group at a time; ask the register
+// allocator for at least LBM_BULK_WAVES wavefronts per SIMD so the memory
+// pipeline has enough loads in flight.
+#ifndef LBM_BULK_WAVES
+#define LBM_BULK_WAVES 4
+#endif
+#ifndef LBM_WAVES_PER_EU
+#define LBM_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
+__global__ __launch_bounds__(BLOCK) LBM_WAVES_PER_EU(M::exact_species ? LBM_BULK_WAVES : 1)
+void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
     long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (c >= P.N) return;
-    bulk_cell<Real, Store, D, NF, NX, PAR>(P, B, c, write_macro);
+    bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }
 
 // family 0: x faces (threads over y,z), 1: y faces (x,z), 2: z faces (x,y)
@@ -247,6 +256,7 @@ struct ImplT final : Impl {
     uint32_t* flist = nullptr;
     long long nflist = 0;
     bool has_emit = false;
+    bool coupled = false;  // parameters match ModelCoupled: run the specialised bulk kernel
     // device-event timing per kernel class
     bool prof_on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[4];
@@ -477,6 +487,16 @@ struct ImplT final : Impl {
             if (rc) return rc;
         }
         if (p.n_reactions > MAXR) return set_err(h_, LBM_ERR_UNSUPPORTED, "too many reactions");
+        {
+            bool sp_const = true;
+            for (int j = 0; j < d.n_species; ++j)
+                if (!d.species_fix[j] && p.species[j].diff_model != LBM_DIFF_CONSTANT) sp_const = false;
+            coupled = D == 3 && NF == 4 && NX == 2 && nf == 4 && nfix == 2 && Q_.thermal && Q_.chem && Q_.poro &&
+                      !Q_.radiation && Q_.force_guo && Q_.eos == EOS_IDEAL && Q_.visc_model == VISC_CONSTANT &&
+                      Q_.poro_model == PORO_DARCY && Q_.tf.diff_model == TDIFF_PRANDTL &&
+                      Q_.tf.cap_model == THERMO_CONSTANT && Q_.ts.diff_model == TDIFF_CONSTANT &&
+                      Q_.ts.cap_model == THERMO_CONSTANT && sp_const && !Q_.lag_tf && !Q_.lag_sp;
+        }
         P = Q_;
         HIP_TRY(hipMemcpyAsync(dP, &P, sizeof(P), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -665,10 +685,7 @@ struct ImplT final : Impl {
             if (D == 2 && p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW)
                 hipLaunchKernelGGL((k_inflow2d<Real, Store>), dim3(1), dim3(BLOCK), 0, stream, dP, B, (Real)p.flow_bc[0]);
             hipEvent_t e_bulk = ev_begin(0);
-            if (stepno & 1)
-                hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, dP, B, wm);
-            else
-                hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, dP, B, wm);
+            launch_bulk((stepno & 1) != 0, wm);
             ev_end(0, e_bulk);
             hipEvent_t e_nee = ev_begin(1);
             if (p.bc_model == LBM_BCMODEL_NEE) {
@@ -696,6 +713,23 @@ struct ImplT final : Impl {
             h_->steps++;
         }
         return LBM_OK;
+    }
+
+    void launch_bulk(bool odd, int wm) {
+        const dim3 grid(grid_for(N)), block(BLOCK);
+        if constexpr (D == 3 && NF == 4 && NX == 2) {
+            if (coupled) {
+                if (odd)
+                    hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelCoupled>), grid, block, 0, stream, dP, B, wm);
+                else
+                    hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelCoupled>), grid, block, 0, stream, dP, B, wm);
+                return;
+            }
+        }
+        if (odd)
+            hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelAny>), grid, block, 0, stream, dP, B, wm);
+        else
+            hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelAny>), grid, block, 0, stream, dP, B, wm);
     }
 
     int reduce(int kind, int comp, double* out) override {

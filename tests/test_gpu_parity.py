@@ -83,3 +83,22 @@ def test_reductions_match_checker(hip, oracle64):
         a, b = A.reduce(_abi.REDUCE_SPECIES_SUM, j), B.reduce(_abi.REDUCE_SPECIES_SUM, j)
         assert abs(a - b) <= 1e-10 * max(1.0, abs(b))
     assert A.get_max_v() == pytest.approx(B.get_max_v(), rel=1e-12)
+
+
+@pytest.mark.parametrize("nz", [16, 64])
+def test_coupled_workload_matches_checker(nz, hip, oracle64):
+    """The 3D coupled pyrolysis setup runs the specialised kernel (compile-time model
+    switches, flow lattice finished before the scalar lattices); nz = 64 also runs the
+    z-face boundary inside the bulk kernel."""
+    from lbm_porouschem_amd import workloads
+    print(f"[parity coupled3d] 12x10x{nz} fp64", flush=True)
+    A = workloads.coupled3d(lib=hip, nx=12, ny=10, nz=nz, precision="fp64")
+    B = workloads.coupled3d(lib=oracle64, nx=12, ny=10, nz=nz, precision="fp64")
+    A.init_simulation()
+    B.init_simulation()
+    for nst in (1, 2, 9):
+        A.run(nst)
+        B.run(nst)
+        fa, fb = fields(A), fields(B)
+        for k in fb:
+            assert rel_l2(fa[k], fb[k]) <= 1e-10, (nz, A.tLattice, k, rel_l2(fa[k], fb[k]))
