@@ -1157,10 +1157,13 @@ LBM_DEV void scatter_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, c
 // ---------------------------------------------------------------------------
 template <class T> LBM_DEV T lane_fetch(T v, int src) { return __shfl(v, src, 64); }
 
-// flow part of the in-wave NEE; returns the neighbour velocity for the scalar part
+// flow part of the in-wave NEE; returns the neighbour velocity for the scalar part and
+// the face cell's post-BC density and velocity (m keeps the pre-BC moments, which the
+// reactions and the interphase exchange of the same step still read)
 template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void nee_inwave_flow(const DevParams<Real>& P, const Bufs<Real, Store>& B, bool isface, int face, long long fc,
-                             int src, Real* __restrict__ f, CellMac<Real, NF, NX>& m, Real& nux, Real& nuy, Real& nuz) {
+                             int src, Real* __restrict__ f, const CellMac<Real, NF, NX>& m, Real& nux, Real& nuy, Real& nuz,
+                             Real& rho_b, Real& ux_b, Real& uy_b, Real& uz_b) {
     using L = Lat<D>;
     const Real nrho = lane_fetch(m.rho, src), nsol = lane_fetch(m.sol, src);
     nux = lane_fetch(m.ux, src);
@@ -1189,12 +1192,10 @@ LBM_DEV void nee_inwave_flow(const DevParams<Real>& P, const Bufs<Real, Store>& 
         const Real fn = lane_fetch(f[k], src);
         if (isface) f[k] = feq<D, Real>(eos, k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(eos, k, nrho, nux, nuy, nuz, enb));
     }
-    if (isface) {
-        m.rho = rho;
-        m.ux = ux;
-        m.uy = uy;
-        m.uz = uz;
-    }
+    rho_b = rho;
+    ux_b = ux;
+    uy_b = uy;
+    uz_b = uz;
 }
 // scalar part of the in-wave NEE (face velocity already in m)
 template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
@@ -1400,13 +1401,20 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
         const int src = (int)(threadIdx.x & 63) + (zface ? (fpos == 0 ? 1 : -1) : 0);
         const int zfid = D == 3 ? (fpos == 0 ? 4 : 5) : (fpos == 0 ? 2 : 3);
         Real nux = 0, nuy = 0, nuz = 0;
+        Real rho_b = m.rho, ux_b = m.ux, uy_b = m.uy, uz_b = m.uz;
         if (anyz) {
             const long long fc = D == 3 ? (long long)g.x * P.ny + g.y : (long long)g.x;
-            nee_inwave_flow<Real, Store, D, NF, NX, M>(P, B, zface, zfid, fc, src, f, m, nux, nuy, nuz);
+            nee_inwave_flow<Real, Store, D, NF, NX, M>(P, B, zface, zfid, fc, src, f, m, nux, nuy, nuz, rho_b, ux_b,
+                                                       uy_b, uz_b);
         }
         if (shell) side_put<L::Q>(B.side, P.nshell, 0, si, f);
         if (!face) {
-            collide_flow<Real, D, NF, NX, M>(P, f, m);
+            CellMac<Real, NF, NX> mb = m;
+            mb.rho = rho_b;
+            mb.ux = ux_b;
+            mb.uy = uy_b;
+            mb.uz = uz_b;
+            collide_flow<Real, D, NF, NX, M>(P, f, mb);
             scatter<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, f);
         }
         // ---- scalar lattices ----
@@ -1437,6 +1445,11 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
             m.rhos = B.rhos[c];
         const Real Tlag = (LBM_MV(chem, P.chem) && LBM_MV(thermal, P.thermal)) ? B.lag[c] : (Real)0;
         macro_scalar<Real, D, NF, NX, M>(P, p, m, Tlag);
+        // from here on the face cell carries its post-BC moments
+        m.rho = rho_b;
+        m.ux = ux_b;
+        m.uy = uy_b;
+        m.uz = uz_b;
         if (anyz) nee_inwave_scalar<Real, Store, D, NF, NX, M>(P, zface, zfid, src, p, m, nux, nuy, nuz);
         if (shell) {
             if (LBM_MV(thermal, P.thermal)) {
