@@ -29,6 +29,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(size, precision):
+    """HBM bytes per k_bulk launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this same workload (counter KiB scaled by a plain-copy calibration), or
+    None when no summary of this workload is committed."""
+    path = os.path.join(ROOT, "profiles", f"k_bulk_coupled3d_{size}_{precision}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch k_bulk"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(size):
     """CPU restatement of the reference algorithm (FP32 build, all host threads),
     timed on a bounded sample of the same workload."""
@@ -149,7 +162,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n, args.precision),
         },
         "cpu_baseline": None,
     }

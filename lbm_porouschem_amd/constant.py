@@ -7,6 +7,7 @@ The values are the same physical data the reference uses
 (src/LBM/util/constant.py:17,19,21-115).
 """
 
+c = 299792458  # speed of light in vacuum, m/s
 R = 8.314441  # universal gas constant, J/(mol K)
 SIGMA = 5.670374419e-8  # Stefan-Boltzmann constant, W/(m^2 K^4)
 

@@ -439,6 +439,16 @@ class LBMSolverBase:
 
     __repr__ = __str__
 
+    def __call__(self):
+        # LBM3D/_core.py:111-112
+        print(self)
+
+    def static_init_kernel(self):
+        """Lattice constants (LBM3D/_initialize.py:25-43).  The velocity sets, weights and
+        opposite-direction tables are compile-time constants of the HIP kernels, so there
+        is nothing to upload; kept so driver code written for the reference still runs."""
+        return None
+
     def description(self):
         return "\n" + str(self) + "\n"
 
