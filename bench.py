@@ -42,6 +42,25 @@ def measured_traffic(size, precision):
     return None
 
 
+def copy_bandwidth(nbytes=2 << 30, reps=5):
+    """Achievable HBM bandwidth on this device: a plain device-to-device copy of
+    `nbytes`, read + write bytes over the event-timed duration (GB/s)."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms / 1e3) / 1e9
+
+
 def cpu_baseline(size):
     """CPU restatement of the reference algorithm (FP32 build, all host threads),
     timed on a bounded sample of the same workload."""
@@ -99,6 +118,8 @@ def main():
     from lbm_porouschem_amd import _abi, workloads
 
     n = args.size
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy bandwidth {copy_gbs:.0f} GB/s (read + write)")
     lib = _abi.load_library()
     log(f"[bench] building coupled3d {n}^3 {args.precision}")
     t0 = time.perf_counter()
@@ -155,6 +176,8 @@ def main():
             "kernel_ms": {"bulk": round(ms_bulk / n_bulk, 4), "nee": round(ms_nee / K, 4),
                           "face": round(ms_face / K, 4), "step_events": round(ms_step / n_step, 4)},
             "step_hbm_frac": round(B * N / (wall / K) / 1e9 / HBM_PEAK_GBS, 4),
+            "copy_GBs": round(copy_gbs, 1),
+            "bulk_frac_of_copy": round(achieved / copy_gbs, 4),
         },
         "roofline": {
             "bound": "hbm",
