@@ -150,6 +150,15 @@ template <class Real> struct SpecP {  // device species order: fluid species fir
 template <class Real> struct ReacP {
     Real A, Ea, b, Tmin;
     Real reac[MAXS], prod[MAXS], rate[MAXS];  // device species order
+    // Derived on the host so the kernels test integer bits instead of comparing
+    // uniform floats per cell (bit j = device species j, live species only):
+    //   order_mask  reac_j > 0: the species' concentration gates the rate
+    //   rate_mask   ... and rate_j != 0: it multiplies the rate
+    //   pow_mask    ... and rate_j != 1: through pow()
+    //   src_mask    prod_j - reac_j != 0: the species receives a source
+    int order_mask, rate_mask, pow_mask, src_mask, has_b;
+    Real dsm[MAXS];  // (prod_j - reac_j) * dt * molemass_j
+    Real dH;         // sum over src_mask of (prod_j - reac_j) * enthalpy_j
 };
 
 template <class Real> struct DevParams {
@@ -538,36 +547,28 @@ LBM_DEV void reactions(const DevParams<Real>& P, CellMac<Real, NF, NX>& m, Real 
             if (T > R.Tmin) {
                 const Real Tg = T + (Real)1e-6;
                 // pow(x, 0) == 1 exactly, so the common b = 0 case skips the pow
-                kr = R.A * (R.b == (Real)0 ? (Real)1 : (Real)pow(Tg, R.b)) * (Real)exp(-R.Ea / Tg / (Real)GAS_R);
+                kr = R.A * (R.has_b ? (Real)pow(Tg, R.b) : (Real)1) * (Real)exp(-R.Ea / Tg / (Real)GAS_R);
             }
         } else {
             kr = R.A;
         }
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            const bool live = j < NF ? j < (M::exact_species ? NF : P.nf) : (j - NF) < (M::exact_species ? NX : P.nfix);
-            if (live && R.reac[j] > 0) {
+            if ((R.order_mask >> j) & 1) {
                 if (m.S[j] > P.tol) {
-                    if (R.rate[j] != 0) {
+                    if ((R.rate_mask >> j) & 1) {
                         Real base = j < NF ? m.S[j] * m.rho / P.sp[j].molemass : m.S[j] / P.sp[j].molemass;
-                        kr *= R.rate[j] == (Real)1 ? base : (Real)pow(base, R.rate[j]);
+                        kr *= ((R.pow_mask >> j) & 1) ? (Real)pow(base, R.rate[j]) : base;
                     }
                 } else {
                     kr = 0;
                 }
             }
         }
-        Real dH = (Real)0.0;
 #pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const bool live = j < NF ? j < (M::exact_species ? NF : P.nf) : (j - NF) < (M::exact_species ? NX : P.nfix);
-            Real coef = -R.reac[j] + R.prod[j];
-            if (live && coef != (Real)0.) {
-                acc[j] += kr * coef * P.dt * P.sp[j].molemass;
-                dH += coef * P.sp[j].enthalpy;
-            }
-        }
-        if (LBM_MV(thermal, P.thermal)) accH += dH * (-kr * P.dt);
+        for (int j = 0; j < NS; ++j)
+            if ((R.src_mask >> j) & 1) acc[j] += kr * R.dsm[j];
+        if (LBM_MV(thermal, P.thermal)) accH += R.dH * (-kr * P.dt);
     }
 #pragma unroll
     for (int j = 0; j < NS; ++j) m.dS[j] = acc[j];
@@ -1542,11 +1543,15 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
 }
 
 // NEE of one face cell b with inner neighbour nb (LBM3D/_boundary.py:211-282,
-// LBM3D/_scalarField.py:83-141).  Reads and writes the side buffer and the
-// macroscopic fields.
+// LBM3D/_scalarField.py:83-141), one lattice group at a time so a launch has a
+// thread per (group, face cell): part 0 = flow (also writes the post-BC rho, u),
+// 1 = TF, 2 = TS, 3 + j = fluid species j.  Every part derives the boundary
+// velocity itself; the flow part's write of u[b] stores the value the other parts
+// derive, so their concurrent reads of u[b] see the same number either way.
+// Reads and writes the side buffer and the macroscopic fields.
 template <class Real, class Store, int D, int NF, int NX, class M = ModelAny>
-LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int face, int bx, int by, int bz, int nx_,
-                      int ny_, int nz_, long long fc) {
+LBM_DEV void nee_part(const DevParams<Real>& P, const Bufs<Real, Store>& B, int part, int face, int bx, int by, int bz,
+                      int nx_, int ny_, int nz_, long long fc) {
     using L = Lat<D>;
     constexpr int Q = L::Q, QS = L::QS;
     const long long N = P.N;
@@ -1554,9 +1559,9 @@ LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int 
     const long long nb = ((long long)nx_ * P.ny + ny_) * P.nz + nz_;
     const long long sb = band_index<D>(bx, by, bz, P.nx, P.ny, P.nz, 2);
     const long long snb = band_index<D>(nx_, ny_, nz_, P.nx, P.ny, P.nz, 2);
-    Real ux = B.u[b], uy = B.u[N + b], uz = B.u[2 * N + b], rho = B.rho[b];
-    const Real nux = B.u[nb], nuy = B.u[N + nb], nuz = B.u[2 * N + nb], nrho = B.rho[nb];
-    const int bv = P.bc_v[face], br = P.bc_rho[face];
+    Real ux = B.u[b], uy = B.u[N + b], uz = B.u[2 * N + b];
+    const Real nux = B.u[nb], nuy = B.u[N + nb], nuz = B.u[2 * N + nb];
+    const int bv = P.bc_v[face];
     if (bv == BC_FIXED) {
         const Real* vp = B.vprof + B.prof_off[face] * 3 + fc * 3;
         ux = vp[0];
@@ -1567,68 +1572,64 @@ LBM_DEV void nee_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, int 
         uy = nuy;
         uz = nuz;
     }
-    if (br == BC_FIXED) {
-        rho = (D == 2 && face == 0) ? B.rprof[B.prof_off[0] + fc] : P.rho_bc[face];
-    } else if (br == BC_ZEROGRAD) {
-        if (!(D == 3 && face == 4)) rho = nrho;
-    }
-    B.rho[b] = rho;
-    B.u[b] = ux;
-    B.u[N + b] = uy;
-    B.u[2 * N + b] = uz;
-    const Real eb = 1 - B.solid[b], enb = 1 - B.solid[nb];
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const Real fn = B.side[(long long)k * P.nshell + snb];
-        B.side[(long long)k * P.nshell + sb] =
-            feq<D, Real>(LBM_MV(eos, P.eos), k, rho, ux, uy, uz, eb) + (fn - feq<D, Real>(LBM_MV(eos, P.eos), k, nrho, nux, nuy, nuz, enb));
-    }
-    if (LBM_MV(chem, P.chem)) {
-#pragma unroll
-        for (int j = 0; j < NF; ++j) {
-            if (j < (M::exact_species ? NF : P.nf)) {
-                const SpecP<Real>& s = P.sp[j];
-                Real Sb = B.S[(long long)j * N + b];
-                const Real Sn = B.S[(long long)j * N + nb];
-                if (s.bc_type[face] == BC_FIXED) Sb = s.bc_value[face];
-                if (s.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
-                B.S[(long long)j * N + b] = Sb;
-                const int base = P.slot_sp + j * QS;
-#pragma unroll
-                for (int k = 0; k < QS; ++k) {
-                    const Real gn = B.side[(long long)(base + k) * P.nshell + snb];
-                    B.side[(long long)(base + k) * P.nshell + sb] =
-                        geq_sp<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_sp<D, Real>(k, Sn, nux, nuy, nuz));
-                }
-            }
+    if (part == 0) {
+        Real rho = B.rho[b];
+        const Real nrho = B.rho[nb];
+        const int br = P.bc_rho[face];
+        if (br == BC_FIXED) {
+            rho = (D == 2 && face == 0) ? B.rprof[B.prof_off[0] + fc] : P.rho_bc[face];
+        } else if (br == BC_ZEROGRAD) {
+            if (!(D == 3 && face == 4)) rho = nrho;
         }
-    }
-    if (LBM_MV(thermal, P.thermal)) {
-        {
-            Real Sb = B.TF[b];
-            const Real Sn = B.TF[nb];
-            if (P.tf.bc_type[face] == BC_FIXED) Sb = P.tf.bc_value[face];
-            if (P.tf.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
-            B.TF[b] = Sb;
+        B.rho[b] = rho;
+        B.u[b] = ux;
+        B.u[N + b] = uy;
+        B.u[2 * N + b] = uz;
+        const Real eb = 1 - B.solid[b], enb = 1 - B.solid[nb];
 #pragma unroll
-            for (int k = 0; k < QS; ++k) {
-                const Real gn = B.side[(long long)(P.slot_tf + k) * P.nshell + snb];
-                B.side[(long long)(P.slot_tf + k) * P.nshell + sb] =
-                    geq_tf<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_tf<D, Real>(k, Sn, nux, nuy, nuz));
-            }
+        for (int k = 0; k < Q; ++k) {
+            const Real fn = B.side[(long long)k * P.nshell + snb];
+            B.side[(long long)k * P.nshell + sb] = feq<D, Real>(LBM_MV(eos, P.eos), k, rho, ux, uy, uz, eb) +
+                                                   (fn - feq<D, Real>(LBM_MV(eos, P.eos), k, nrho, nux, nuy, nuz, enb));
         }
-        {
-            Real Sb = B.TS[b];
-            const Real Sn = B.TS[nb];
-            if (P.ts.bc_type[face] == BC_FIXED) Sb = P.ts.bc_value[face];
-            if (P.ts.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
-            B.TS[b] = Sb;
+    } else if (part == 1) {
+        Real Sb = B.TF[b];
+        const Real Sn = B.TF[nb];
+        if (P.tf.bc_type[face] == BC_FIXED) Sb = P.tf.bc_value[face];
+        if (P.tf.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
+        B.TF[b] = Sb;
 #pragma unroll
-            for (int k = 0; k < QS; ++k) {
-                const Real gn = B.side[(long long)(P.slot_ts + k) * P.nshell + snb];
-                const Real w = (Real)L::ws(k);
-                B.side[(long long)(P.slot_ts + k) * P.nshell + sb] = w * Sb + (gn - w * Sn);
-            }
+        for (int k = 0; k < QS; ++k) {
+            const Real gn = B.side[(long long)(P.slot_tf + k) * P.nshell + snb];
+            B.side[(long long)(P.slot_tf + k) * P.nshell + sb] =
+                geq_tf<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_tf<D, Real>(k, Sn, nux, nuy, nuz));
+        }
+    } else if (part == 2) {
+        Real Sb = B.TS[b];
+        const Real Sn = B.TS[nb];
+        if (P.ts.bc_type[face] == BC_FIXED) Sb = P.ts.bc_value[face];
+        if (P.ts.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
+        B.TS[b] = Sb;
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            const Real gn = B.side[(long long)(P.slot_ts + k) * P.nshell + snb];
+            const Real w = (Real)L::ws(k);
+            B.side[(long long)(P.slot_ts + k) * P.nshell + sb] = w * Sb + (gn - w * Sn);
+        }
+    } else {
+        const int j = part - 3;
+        const SpecP<Real>& sp = P.sp[j];
+        Real Sb = B.S[(long long)j * N + b];
+        const Real Sn = B.S[(long long)j * N + nb];
+        if (sp.bc_type[face] == BC_FIXED) Sb = sp.bc_value[face];
+        if (sp.bc_type[face] == BC_ZEROGRAD) Sb = Sn;
+        B.S[(long long)j * N + b] = Sb;
+        const int base = P.slot_sp + j * QS;
+#pragma unroll
+        for (int k = 0; k < QS; ++k) {
+            const Real gn = B.side[(long long)(base + k) * P.nshell + snb];
+            B.side[(long long)(base + k) * P.nshell + sb] =
+                geq_sp<D, Real>(k, Sb, ux, uy, uz) + (gn - geq_sp<D, Real>(k, Sn, nux, nuy, nuz));
         }
     }
 }

@@ -67,34 +67,67 @@ void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int wri
     bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }
 
-// family 0: x faces (threads over y,z), 1: y faces (x,z), 2: z faces (x,y)
+// NEE of one face family (0: x faces, 1: y faces, 2: z faces) with a thread per
+// (lattice group, face side, face cell).  Both faces of the axis run in one launch:
+// every axis has >= 4 cells (lbm_create), so no face cell is the other face's inner
+// neighbour.  Thread order: group slowest, then side, then the face cells with the
+// fast axis fastest (coalesced side-buffer rows).
+// The fast-axis faces (z in 3D, y in 2D) run over `rows`, the list of rows whose
+// face cells are not handled inside the bulk kernel.
 template <class Real, class Store, int D, int NF, int NX>
-__global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int family) {
+__global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int family,
+                                               int nparts, const uint32_t* __restrict__ rows, long long nrows) {
     const DevParams<Real>& P = *Pp;
-    long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    const long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const int nx = P.nx, ny = P.ny, nz = P.nz;
-    if (family == 0) {
-        if (t >= (long long)ny * nz) return;
-        int y = (int)(t / nz), z = (int)(t % nz);
-        long long fc = (long long)y * nz + z;
-        nee_cell<Real, Store, D, NF, NX>(P, B, 0, 0, y, z, 1, y, z, fc);
-        nee_cell<Real, Store, D, NF, NX>(P, B, 1, nx - 1, y, z, nx - 2, y, z, fc);
-    } else if (family == 1) {
-        if (t >= (long long)nx * nz) return;
-        int x = (int)(t / nz), z = (int)(t % nz);
-        // 2D: the y faces are the fast-axis faces; rows handled inside the bulk kernel are skipped
-        if (D == 2 && (B.mask[(long long)x * ny] & ZFAST_BIT)) return;
-        long long fc = (long long)x * nz + z;
-        nee_cell<Real, Store, D, NF, NX>(P, B, 2, x, 0, z, x, 1, z, fc);
-        nee_cell<Real, Store, D, NF, NX>(P, B, 3, x, ny - 1, z, x, ny - 2, z, fc);
+    const bool fastfam = family == (D == 3 ? 2 : 1);
+    const long long ncell = fastfam ? nrows : (family == 0 ? (long long)ny * nz : (long long)nx * nz);
+    if (t >= ncell * 2 * nparts) return;
+    const long long cell = t % ncell;
+    const int side = (int)((t / ncell) % 2);
+    const int part = (int)(t / (ncell * 2));
+    const int face = 2 * family + side;
+    int bx, by, bz, ix, iy, iz;
+    long long fc;
+    if (fastfam) {
+        const uint32_t r = rows[cell];
+        if (D == 3) {
+            bx = (int)(r / ny);
+            by = (int)(r % ny);
+            bz = side ? nz - 1 : 0;
+            ix = bx;
+            iy = by;
+            iz = side ? nz - 2 : 1;
+        } else {
+            bx = (int)r;
+            by = side ? ny - 1 : 0;
+            bz = 0;
+            ix = bx;
+            iy = side ? ny - 2 : 1;
+            iz = 0;
+        }
+        fc = r;
+    } else if (family == 0) {
+        by = (int)(cell / nz);
+        bz = (int)(cell % nz);
+        bx = side ? nx - 1 : 0;
+        ix = side ? nx - 2 : 1;
+        iy = by;
+        iz = bz;
+        fc = cell;
     } else {
-        if (t >= (long long)nx * ny) return;
-        int x = (int)(t / ny), y = (int)(t % ny);
-        if (B.mask[((long long)x * ny + y) * nz] & ZFAST_BIT) return;
-        long long fc = (long long)x * ny + y;
-        nee_cell<Real, Store, D, NF, NX>(P, B, 4, x, y, 0, x, y, 1, fc);
-        nee_cell<Real, Store, D, NF, NX>(P, B, 5, x, y, nz - 1, x, y, nz - 2, fc);
+        bx = (int)(cell / nz);
+        bz = (int)(cell % nz);
+        by = side ? ny - 1 : 0;
+        iy = side ? ny - 2 : 1;
+        ix = bx;
+        iz = bz;
+        fc = cell;
     }
+    // parts: 0 flow, then TF and TS when thermal, then the fluid species
+    int kind = part;
+    if (!P.thermal && part >= 1) kind = part + 2;
+    nee_part<Real, Store, D, NF, NX>(P, B, kind, face, bx, by, bz, ix, iy, iz, fc);
 }
 
 template <class Real, class Store, int D, int NF, int NX, int PAR>
@@ -244,7 +277,8 @@ struct ImplT final : Impl {
     int nx, ny, nz;
     long long N, nshell, nface;
     int nf = 0, nfix = 0;
-    int dev_of[LBM_MAX_SPECIES];  // registration order -> device order
+    int dev_of[LBM_MAX_SPECIES];  // registration order -> device field order (fluid first, then FIX)
+    int tpl_of[LBM_MAX_SPECIES];  // registration order -> kernel index (fluid j, FIX NF + j)
     DevParams<Real> P{};
     DevParams<Real>* dP = nullptr;
     Bufs<Real, Store> B{};
@@ -255,6 +289,8 @@ struct ImplT final : Impl {
     double* partials = nullptr;
     uint32_t* flist = nullptr;
     long long nflist = 0;
+    uint32_t* frows = nullptr;  // fast-axis face rows left to k_nee (3D: x*ny+y, 2D: x)
+    long long nfrows = 0;
     bool has_emit = false;
     bool coupled = false;  // parameters match ModelCoupled: run the specialised bulk kernel
     // device-event timing per kernel class
@@ -288,9 +324,12 @@ struct ImplT final : Impl {
         nshell = band_count<D>(nx, ny, nz, 2);
         nface = band_count<D>(nx, ny, nz, 1);
         for (int j = 0; j < h->d.n_species; ++j)
-            if (!h->d.species_fix[j]) dev_of[j] = nf++;
+            if (!h->d.species_fix[j]) tpl_of[j] = dev_of[j] = nf++;
         for (int j = 0; j < h->d.n_species; ++j)
-            if (h->d.species_fix[j]) dev_of[j] = nf + nfix++;
+            if (h->d.species_fix[j]) {
+                tpl_of[j] = NF + nfix;
+                dev_of[j] = nf + nfix++;
+            }
     }
     ~ImplT() override {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -440,7 +479,7 @@ struct ImplT final : Impl {
         bool any_sp_suth = false;
         for (int j = 0; j < d.n_species; ++j) {
             const lbm_species_params& s = p.species[j];
-            SpecP<Real>& t = Q_.sp[dev_of[j]];
+            SpecP<Real>& t = Q_.sp[tpl_of[j]];
             t.visc_model = s.visc_model;
             t.diff_model = s.diff_model;
             t.cond_model = s.cond_model;
@@ -469,9 +508,29 @@ struct ImplT final : Impl {
             t.b = (Real)s.b;
             t.Tmin = (Real)s.Tmin;
             for (int j = 0; j < d.n_species; ++j) {
-                t.reac[dev_of[j]] = (Real)s.coef_reactant[j];
-                t.prod[dev_of[j]] = (Real)s.coef_product[j];
-                t.rate[dev_of[j]] = (Real)s.coef_rate[j];
+                t.reac[tpl_of[j]] = (Real)s.coef_reactant[j];
+                t.prod[tpl_of[j]] = (Real)s.coef_product[j];
+                t.rate[tpl_of[j]] = (Real)s.coef_rate[j];
+            }
+            // integer gates and per-reaction constants of the rate and source (bit = device species)
+            t.has_b = t.b != (Real)0;
+            t.dH = (Real)0;
+            for (int jd = 0; jd < NF + NX; ++jd) {
+                if (jd < NF ? jd >= nf : jd - NF >= nfix) continue;
+                const int bit = 1 << jd;
+                if (t.reac[jd] > 0) {
+                    t.order_mask |= bit;
+                    if (t.rate[jd] != 0) {
+                        t.rate_mask |= bit;
+                        if (t.rate[jd] != (Real)1) t.pow_mask |= bit;
+                    }
+                }
+                const Real coef = -t.reac[jd] + t.prod[jd];
+                if (coef != (Real)0) {
+                    t.src_mask |= bit;
+                    t.dsm[jd] = coef * Q_.dt * Q_.sp[jd].molemass;
+                    t.dH += coef * Q_.sp[jd].enthalpy;
+                }
             }
         }
         // old-state reads of the porous velocity correction (LBM3D/_evolution.py:130-150)
@@ -647,6 +706,20 @@ struct ImplT final : Impl {
                 const long long c = ((long long)x * ny + y) * nz + z;
                 if (mh[(size_t)c] & FACE_BIT) fl.push_back((uint32_t)c);
             }
+            // rows whose fast-axis face cells k_nee handles (the bulk kernel does the
+            // rows with ZFAST set on their first cell)
+            std::vector<uint32_t> fr;
+            const long long nrow = D == 3 ? (long long)nx * ny : (long long)nx;
+            const long long rstride = D == 3 ? nz : ny;
+            for (long long r = 0; r < nrow; ++r)
+                if (!(mh[(size_t)(r * rstride)] & ZFAST_BIT)) fr.push_back((uint32_t)r);
+            nfrows = (long long)fr.size();
+            if (!frows) {
+                int rc = dalloc(&frows, (size_t)nrow);
+                if (rc) return rc;
+            }
+            if (nfrows)
+                HIP_TRY(hipMemcpyAsync(frows, fr.data(), fr.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
             nflist = (long long)fl.size();
             if (!flist) {
                 int rc = dalloc(&flist, (size_t)nface);
@@ -689,13 +762,14 @@ struct ImplT final : Impl {
             ev_end(0, e_bulk);
             hipEvent_t e_nee = ev_begin(1);
             if (p.bc_model == LBM_BCMODEL_NEE) {
-                hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for((long long)ny * nz)), dim3(BLOCK), 0,
-                                   stream, dP, B, 0);
-                hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for((long long)nx * nz)), dim3(BLOCK), 0,
-                                   stream, dP, B, 1);
-                if (D == 3)
-                    hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for((long long)nx * ny)), dim3(BLOCK),
-                                       0, stream, dP, B, 2);
+                const int nparts = 1 + (h_->d.thermal ? 2 : 0) + (h_->d.chemical ? nf : 0);
+                for (int fam = 0; fam < D; ++fam) {
+                    const bool fastfam = fam == D - 1;
+                    const long long ncell = fastfam ? nfrows : (fam == 0 ? (long long)ny * nz : (long long)nx * nz);
+                    if (ncell == 0) continue;
+                    hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for(ncell * 2 * nparts)), dim3(BLOCK), 0,
+                                       stream, dP, B, fam, nparts, (const uint32_t*)frows, nfrows);
+                }
             }
             ev_end(1, e_nee);
             hipEvent_t e_face = ev_begin(2);

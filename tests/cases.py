@@ -9,8 +9,11 @@ SPECIES = (("vol", False, 30.0), ("tar", False, 60.0), ("gas", False, 20.0), ("i
            ("biomass", True, 100.0), ("char", True, 12.0))
 
 
+PARTIAL_SPECIES = (("vol", False, 30.0), ("gas", False, 20.0), ("biomass", True, 100.0))
+
+
 def build(lib, dim, n, therm=True, chem=False, poro=True, prec="fp64", solidmode="gray", seed=0,
-          poro_model="darcy", visc="constant"):
+          poro_model="darcy", visc="constant", species="full"):
     rng = np.random.default_rng(seed)
     if dim == 2:
         lb = LBM2DSolver(n[0], n[1], 1, dx=1, dt=1, isThermal=therm, isChemical=chem, isPoro=poro, precision=prec,
@@ -63,19 +66,29 @@ def build(lib, dim, n, therm=True, chem=False, poro=True, prec="fp64", solidmode
         elif poro_model == "df":
             lb.set_poro_Darcy_Forchheimer(s > 0, 0.05, 0.1)
     if chem:
-        for name, fix, mm in SPECIES:
+        table = SPECIES if species == "full" else PARTIAL_SPECIES
+        for name, fix, mm in table:
             lb.set_specie(name, fix, mm)
             if not fix:
-                lb.set_specie_diff(name, 0.1)
-        lb.init_specie("inert", 1.0)
+                lb.set_specie_diff(name, 0.1 if name != "gas" else 0.07)
         lb.init_specie("biomass", (s > 0) * 1.0)
         # k(800 K) * dt of order 1e-3: the explicit source stays in its stable range
-        lb.add_reaction("biomass => tar", 3.0, 2.0e4)
-        lb.add_reaction("biomass => gas", 1.0, 1.5e4)
-        lb.add_reaction("biomass => 0.3 char + 0.7 vol", 2.0, 1.8e4)
+        if species == "full":
+            lb.init_specie("inert", 1.0)
+            lb.add_reaction("biomass => tar", 3.0, 2.0e4)
+            lb.add_reaction("biomass => gas", 1.0, 1.5e4)
+            lb.add_reaction("biomass => 0.3 char + 0.7 vol", 2.0, 1.8e4)
+            lb.set_specie_BC("inert", 0, BC.fixedValue)
+            lb.set_specie_BC_value("inert", 0, 1.0)
+        else:
+            # fewer species than the kernel's compile-time maximum; a second-order gas
+            # reactant exercises the pow() branch of the rate
+            lb.init_specie("vol", 1.0)
+            lb.add_reaction("biomass => gas", 1.0, 1.5e4)
+            lb.add_reaction("2 vol + biomass => gas", 40.0, 1.8e4)
+            lb.set_specie_BC("vol", 0, BC.fixedValue)
+            lb.set_specie_BC_value("vol", 0, 1.0)
         lb.set_species_BC(1, BC.zeroGradient)
-        lb.set_specie_BC("inert", 0, BC.fixedValue)
-        lb.set_specie_BC_value("inert", 0, 1.0)
         if visc == "mixture":
             lb.set_viscosity_mixture()
             for name, fix, _ in SPECIES:

@@ -22,6 +22,9 @@ CASES = {
     "3d_chem_mixed_solid": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, solidmode="mixed"),
     "3d_chem_df": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, poro_model="df"),
     "3d_spherical": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, poro_model="spherical"),
+    # two fluid species and one FIX species in the kernel built for four and two
+    "3d_chem_partial_species": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, species="partial"),
+    "2d_chem_partial_species": dict(dim=2, n=(24, 16), therm=True, poro=True, chem=True, species="partial"),
     # fast axis a multiple of 64: the z-face (2D: y-face) boundary runs inside the bulk kernel
     "3d_fastface_chem": dict(dim=3, n=(9, 8, 64), therm=True, poro=True, chem=True),
     "3d_fastface_mixed_solid": dict(dim=3, n=(9, 8, 64), therm=True, poro=True, chem=True, solidmode="mixed"),
