@@ -29,11 +29,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(size, precision):
+def measured_traffic(size, precision, collision):
     """HBM bytes per k_bulk launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this same workload (counter KiB scaled by a plain-copy calibration), or
     None when no summary of this workload is committed."""
-    path = os.path.join(ROOT, "profiles", f"k_bulk_coupled3d_{size}_{precision}.txt")
+    path = os.path.join(ROOT, "profiles", f"k_bulk_coupled3d_{size}_{precision}_{collision}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -61,14 +61,14 @@ def copy_bandwidth(nbytes=2 << 30, reps=5):
     return 2 * nbytes / (ms / 1e3) / 1e9
 
 
-def cpu_baseline(size):
+def cpu_baseline(size, collision):
     """CPU restatement of the reference algorithm (FP32 build, all host threads),
     timed on a bounded sample of the same workload."""
     from lbm_porouschem_amd import _abi, workloads
     path = os.path.join(ROOT, "oracle", "liblbm_oracle_f32.so")
     lib = _abi.CLib(path)
     nx, ny, nz = size
-    lb = workloads.coupled3d(lib=lib, nx=nx, ny=ny, nz=nz, precision="fp32")
+    lb = workloads.coupled3d(lib=lib, nx=nx, ny=ny, nz=nz, precision="fp32", collision=collision)
     lb.init_simulation()
     lb.run(1)
     steps = 4
@@ -81,7 +81,7 @@ def cpu_baseline(size):
         "unit": "MLUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"coupled3d {nx}x{ny}x{nz} fp32, {steps} steps after 1 warm-up step, CPU restatement "
+        "sample": f"coupled3d {nx}x{ny}x{nz} fp32 {collision}, {steps} steps after 1 warm-up step, CPU restatement "
                   f"(oracle/lbm_oracle.c, two-lattice AoS push like the reference), {threads} OpenMP threads",
     }
 
@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--collision", default="mrt", choices=["mrt", "bgk"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this many MiB (scale reference for rocprofv3 "
@@ -123,7 +124,7 @@ def main():
     lib = _abi.load_library()
     log(f"[bench] building coupled3d {n}^3 {args.precision}")
     t0 = time.perf_counter()
-    lb = workloads.coupled3d(lib=lib, nx=n, ny=n, nz=n, precision=args.precision)
+    lb = workloads.coupled3d(lib=lib, nx=n, ny=n, nz=n, precision=args.precision, collision=args.collision)
     lb.init_simulation()
     h = lb._handle
     log(f"[bench] init done in {time.perf_counter() - t0:.1f}s; warmup {args.warmup}")
@@ -168,8 +169,10 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic (seeded gray sphere packing, seeded Arrhenius parameters)",
         "config": {
-            "workload": f"coupled3d {n}x{n}x{n} {args.precision} BGK: D3Q19 flow + D3Q7 TF + D3Q7 TS + 4 D3Q7 "
-                        "species + 2 FIX species, 3 pyrolysis reactions, Darcy gray spheres r 6-12 solid 0.5",
+            "workload": f"coupled3d {n}x{n}x{n} {args.precision} {args.collision.upper()}: D3Q19 flow + D3Q7 TF + "
+                        "D3Q7 TS + 4 D3Q7 species + 2 FIX species, 3 pyrolysis reactions, Darcy gray spheres r 6-12 "
+                        "solid 0.5",
+            "collision": args.collision,
             "cells": N,
             "populations_per_cell": workloads.COUPLED3D_Q,
             "algorithmic_bytes_per_cell_step": B,
@@ -185,14 +188,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n, args.precision),
+            "traffic": measured_traffic(n, args.precision, args.collision),
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (bounded sample)")
         del lb
-        result["cpu_baseline"] = cpu_baseline((n, n // 2, n // 2))
+        result["cpu_baseline"] = cpu_baseline((n, n // 2, n // 2), args.collision)
     print(json.dumps(result), flush=True)
 
 

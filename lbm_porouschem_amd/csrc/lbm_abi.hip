@@ -41,7 +41,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out) {
     if (!h) return LBM_ERR_OOM;
     h->d = *desc;
     h->err[0] = 0;
-    if (desc->collision != LBM_COLL_BGK) {
+    if (desc->collision != LBM_COLL_BGK && desc->collision != LBM_COLL_MRT) {
         delete h;
         return LBM_ERR_UNSUPPORTED;
     }

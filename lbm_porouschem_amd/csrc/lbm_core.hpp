@@ -34,6 +34,8 @@
 #include <stdint.h>
 #include <math.h>
 
+#include <utility>
+
 #ifndef LBM_DEV
 #define LBM_DEV __device__ __forceinline__
 #endif
@@ -56,6 +58,7 @@ enum { THERMO_NONE = 0, THERMO_CONSTANT = 1, THERMO_POLY = 2, THERMO_NASA7 = 3, 
 enum { TDIFF_NONE = 0, TDIFF_CONSTANT = 1, TDIFF_DERIVED = 2, TDIFF_PRANDTL = 3 };
 enum { DIFF_NONE = 0, DIFF_CONSTANT = 1, DIFF_SCHMIDT = 2 };
 enum { EOS_IDEAL = 0, EOS_INCOMP = 1 };
+enum { COLL_BGK = 0, COLL_MRT = 1 };
 enum { BC_PERIODIC = 0, BC_ZEROGRAD = 1, BC_FIXED = 2 };
 
 // ---------------------------------------------------------------------------
@@ -73,15 +76,17 @@ struct ModelAny {
     static constexpr int thermal = -1, chem = -1, poro = -1, radiation = -1, force_guo = -1, eos = -1, visc = -1,
                          poro_model = -1, tf_diff = -1, tf_cond = -1, tf_cap = -1, ts_diff = -1, ts_cond = -1,
                          ts_cap = -1, sp_visc = -1, sp_diff = -1, sp_cond = -1, lag_tf = -1, lag_sp = -1,
-                         exact_species = 0;
+                         coll = -1, exact_species = 0;
 };
-struct ModelCoupled {
+template <int COLL> struct ModelCoupledT {
     static constexpr int thermal = 1, chem = 1, poro = 1, radiation = 0, force_guo = 1, eos = EOS_IDEAL,
                          visc = VISC_CONSTANT, poro_model = PORO_DARCY, tf_diff = TDIFF_PRANDTL, tf_cond = -1,
                          tf_cap = THERMO_CONSTANT, ts_diff = TDIFF_CONSTANT, ts_cond = -1, ts_cap = THERMO_CONSTANT,
                          sp_visc = -1, sp_diff = DIFF_CONSTANT, sp_cond = -1, lag_tf = 0, lag_sp = 0,
-                         exact_species = 1;
+                         coll = COLL, exact_species = 1;
 };
+using ModelCoupled = ModelCoupledT<COLL_BGK>;
+using ModelCoupledMRT = ModelCoupledT<COLL_MRT>;
 #define LBM_MV(key, runtime) (M::key >= 0 ? M::key : (runtime))
 
 
@@ -167,6 +172,9 @@ template <class Real> struct DevParams {
     int thermal, chem, poro, radiation;
     int nf, nfix;             // species counts in use (<= template maxima)
     int eos, visc_model, force_guo, poro_model, rad_model, bc_nee;
+    int coll;                 // COLL_BGK / COLL_MRT
+    int mrt_tau_mask;         // bit r: moment r relaxes with 1/tau
+    Real mrt_s[19];           // fixed MRT rates of the other moments
     int bc_v[6], bc_rho[6];
     int lag_tf, lag_sp;       // extra lagged fields needed by the porous correction
     int n_reac;
@@ -749,6 +757,149 @@ template <class Real, class M = ModelAny> LBM_DEV Real sp_D(const DevParams<Real
 //   scalar g* = (1 - 1/tau_s) g + w_k P_k (S/tau_s + dS),  P_k = geq_k / (w_k S)
 //          (equilibrium and source share P_k because both use the cell velocity)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// MRT collision (opt-in; the reference is BGK only).  Orthogonal moment bases,
+// polynomial rows of the lattice velocity:
+//   D3Q19 d'Humieres et al., Phil. Trans. R. Soc. A 360 (2002) 437: rho, e, eps, jx,
+//         qx, jy, qy, jz, qz, 3pxx, 3pixx, pww, piww, pxy, pyz, pxz, mx, my, mz
+//   D2Q9  Lallemand & Luo, Phys. Rev. E 61 (2000) 6546: rho, e, eps, jx, qx, jy, qy,
+//         pxx, pxy
+// m*_r = m_r - s_r (m_r - m_eq_r) + (1 - s_r/2) m_F_r and f* = M^-1 m*, where M^-1
+// = M^T diag(1/|row|^2) by orthogonality.  The equilibrium and Guo-forcing moments
+// come in closed form from lattice sums of the basis rows (the same quantities as
+// transforming feq_k and the forcing term, without the second and third transform).
+// ---------------------------------------------------------------------------
+template <int D> LBM_DEV constexpr double mrt_m(int r, int k) {
+    using L = Lat<D>;
+    const double x = L::ex(k), y = L::ey(k), z = L::ez(k);
+    const double e2 = x * x + y * y + z * z;
+    if (D == 2) {
+        return r == 0 ? 1.0
+             : r == 1 ? -4 + 3 * e2
+             : r == 2 ? 4 - 10.5 * e2 + 4.5 * e2 * e2
+             : r == 3 ? x
+             : r == 4 ? (-5 + 3 * e2) * x
+             : r == 5 ? y
+             : r == 6 ? (-5 + 3 * e2) * y
+             : r == 7 ? x * x - y * y
+             : x * y;
+    }
+    return r == 0 ? 1.0
+         : r == 1 ? 19 * e2 - 30
+         : r == 2 ? (21 * e2 * e2 - 53 * e2 + 24) / 2
+         : r == 3 ? x
+         : r == 4 ? (5 * e2 - 9) * x
+         : r == 5 ? y
+         : r == 6 ? (5 * e2 - 9) * y
+         : r == 7 ? z
+         : r == 8 ? (5 * e2 - 9) * z
+         : r == 9 ? 3 * x * x - e2
+         : r == 10 ? (3 * e2 - 5) * (3 * x * x - e2)
+         : r == 11 ? y * y - z * z
+         : r == 12 ? (3 * e2 - 5) * (y * y - z * z)
+         : r == 13 ? x * y
+         : r == 14 ? y * z
+         : r == 15 ? x * z
+         : r == 16 ? (y * y - z * z) * x
+         : r == 17 ? (z * z - x * x) * y
+         : (x * x - y * y) * z;
+}
+template <int D> LBM_DEV constexpr double mrt_e(int k, int a) {
+    return a == 0 ? Lat<D>::ex(k) : (a == 1 ? Lat<D>::ey(k) : Lat<D>::ez(k));
+}
+// sum_k m_rk^2, sum_k w_k m_rk, sum_k w_k m_rk e_a, sum_k w_k m_rk e_a e_b
+template <int D> LBM_DEV constexpr double mrt_norm(int r) {
+    double s = 0;
+    for (int k = 0; k < Lat<D>::Q; ++k) s += mrt_m<D>(r, k) * mrt_m<D>(r, k);
+    return s;
+}
+template <int D> LBM_DEV constexpr double mrt_s0(int r) {
+    double s = 0;
+    for (int k = 0; k < Lat<D>::Q; ++k) s += Lat<D>::w(k) * mrt_m<D>(r, k);
+    return s;
+}
+template <int D> LBM_DEV constexpr double mrt_s1(int r, int a) {
+    double s = 0;
+    for (int k = 0; k < Lat<D>::Q; ++k) s += Lat<D>::w(k) * mrt_m<D>(r, k) * mrt_e<D>(k, a);
+    return s;
+}
+template <int D> LBM_DEV constexpr double mrt_s2(int r, int a, int b) {
+    double s = 0;
+    for (int k = 0; k < Lat<D>::Q; ++k) s += Lat<D>::w(k) * mrt_m<D>(r, k) * mrt_e<D>(k, a) * mrt_e<D>(k, b);
+    return s;
+}
+// rounds tiny lattice-sum residues (1e-17 of a 1/36 weight sum) to exact zeros
+LBM_DEV constexpr double mrt_clean(double v) { return (v < 1e-12 && v > -1e-12) ? 0.0 : v; }
+
+template <class F, int... I> LBM_DEV void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> LBM_DEV void static_for(F&& f) { static_for_seq(f, std::make_integer_sequence<int, N>{}); }
+
+// f: gathered populations, replaced by the post-collision ones.  a0..a2: feq_k / w_k =
+// a0 + a1 (e.u) + a2 (e.u)^2; c0..c2: forcing shape G_k / w_k = c0 + c1 (e.F) + c2 (e.u)(e.F).
+template <class Real, int D>
+LBM_DEV void mrt_relax(const DevParams<Real>& P, Real* __restrict__ f, Real itau, Real a0, Real a1, Real a2, Real c0,
+                       Real c1, Real c2, Real ux, Real uy, Real uz, Real Fx, Real Fy, Real Fz) {
+    using L = Lat<D>;
+    constexpr int Q = L::Q;
+    const Real u[3] = {ux, uy, uz}, F[3] = {Fx, Fy, Fz};
+    Real y[Q];
+    y[0] = (Real)0;  // density: m_0 = m_eq_0 and the forcing moment vanishes
+    static_for<Q>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (r > 0) {
+            Real mr = (Real)0;
+            static_for<Q>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr double c = mrt_m<D>(r, k);
+                if constexpr (c == 1.0) mr += f[k];
+                else if constexpr (c == -1.0) mr -= f[k];
+                else if constexpr (c != 0.0) mr += (Real)c * f[k];
+            });
+            constexpr double s0 = mrt_clean(mrt_s0<D>(r));
+            Real meq = (Real)0, mF = (Real)0;
+            if constexpr (s0 != 0.0) {
+                meq = (Real)s0 * a0;
+                mF = (Real)s0 * c0;
+            }
+            Real lin_u = 0, lin_F = 0, quad_u = 0, quad_F = 0;
+            static_for<D>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                constexpr double s1 = mrt_clean(mrt_s1<D>(r, a));
+                if constexpr (s1 != 0.0) {
+                    lin_u += (Real)s1 * u[a];
+                    lin_F += (Real)s1 * F[a];
+                }
+                static_for<D>([&](auto bc) {
+                    constexpr int b = decltype(bc)::value;
+                    constexpr double s2 = mrt_clean(mrt_s2<D>(r, a, b));
+                    if constexpr (s2 != 0.0) {
+                        quad_u += (Real)s2 * u[a] * u[b];
+                        quad_F += (Real)s2 * u[a] * F[b];
+                    }
+                });
+            });
+            meq += a1 * lin_u + a2 * quad_u;
+            mF += c1 * lin_F + c2 * quad_F;
+            const Real sr = ((P.mrt_tau_mask >> r) & 1) ? itau : P.mrt_s[r];
+            y[r] = sr * (mr - meq) - (1 - (Real)0.5 * sr) * mF;
+        }
+    });
+    static_for<Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        Real d = (Real)0;
+        static_for<Q>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (r > 0) {
+                constexpr double c = mrt_m<D>(r, k) / mrt_norm<D>(r);
+                if constexpr (c != 0.0) d += (Real)c * y[r];
+            }
+        });
+        f[k] -= d;
+    });
+}
+
 template <class Real, int D, int NF, int NX, class M = ModelAny>
 LBM_DEV void collide_flow(const DevParams<Real>& P, Real* __restrict__ f, const CellMac<Real, NF, NX>& m) {
     using L = Lat<D>;
@@ -796,6 +947,12 @@ LBM_DEV void collide_flow(const DevParams<Real>& P, Real* __restrict__ f, const 
     const Real gpre = guo ? (incomp ? (1 - (Real)0.5 * itau) : (1 - (Real)0.5 * itau) * rho) : (Real)0;
     const Real g3 = 3 * gpre, g9 = 9 * gpre * iee;
     const Real uoF = (ux * Fx + uy * Fy + uz * Fz) * iee;
+    if (LBM_MV(coll, P.coll) == COLL_MRT) {
+        // forcing shape without the (1 - 1/(2 tau)) factor of the BGK form
+        const Real gh = guo ? (incomp ? (Real)1 : rho) : (Real)0;
+        mrt_relax<Real, D>(P, f, itau, a0, a1, a2, -3 * gh * uoF, 3 * gh, 9 * gh * iee, ux, uy, uz, Fx, Fy, Fz);
+        return;
+    }
     const Real b0 = itau * a0 - g3 * uoF, b1 = itau * a1, b2 = itau * a2;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {

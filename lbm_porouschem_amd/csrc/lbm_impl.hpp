@@ -435,6 +435,12 @@ struct ImplT final : Impl {
         Q_.poro_model = p.poro_model;
         Q_.rad_model = p.radiation_model;
         Q_.bc_nee = p.bc_model == LBM_BCMODEL_NEE;
+        Q_.coll = d.collision == LBM_COLL_MRT ? COLL_MRT : COLL_BGK;
+        Q_.mrt_tau_mask = 0;
+        for (int r = 0; r < 19; ++r) {
+            Q_.mrt_s[r] = p.mrt_rates[r] > 0 ? (Real)p.mrt_rates[r] : (Real)0;
+            if (!(p.mrt_rates[r] > 0)) Q_.mrt_tau_mask |= 1 << r;
+        }
         for (int f = 0; f < 6; ++f) {
             Q_.bc_v[f] = p.bc_v[f];
             Q_.bc_rho[f] = p.bc_rho[f];
@@ -793,10 +799,21 @@ struct ImplT final : Impl {
         const dim3 grid(grid_for(N)), block(BLOCK);
         if constexpr (D == 3 && NF == 4 && NX == 2) {
             if (coupled) {
-                if (odd)
-                    hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelCoupled>), grid, block, 0, stream, dP, B, wm);
-                else
-                    hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelCoupled>), grid, block, 0, stream, dP, B, wm);
+                if (P.coll == COLL_MRT) {
+                    if (odd)
+                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelCoupledMRT>), grid, block, 0, stream, dP,
+                                           B, wm);
+                    else
+                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelCoupledMRT>), grid, block, 0, stream, dP,
+                                           B, wm);
+                } else {
+                    if (odd)
+                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelCoupled>), grid, block, 0, stream, dP, B,
+                                           wm);
+                    else
+                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelCoupled>), grid, block, 0, stream, dP, B,
+                                           wm);
+                }
                 return;
             }
         }

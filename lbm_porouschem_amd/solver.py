@@ -365,6 +365,8 @@ class LBMSolverBase:
         self.boundary_condition_model = BC_MODEL.NEE
         self.EOS = FLUID_STATE_EQUATION.IDEAL_GAS
         self.max_velocity = 0.0  # > 0 enables the divergence detector of lbm_step
+        # MRT relaxation rate per moment (collision="mrt"); <= 0 relaxes that moment with 1/tau
+        self.mrt_rates = [0.0] * 19
         nf = self._nfaces
         self.rho = self._cell_field(_abi.FIELD_RHO, writable=False)
         self.v = self._cell_field(_abi.FIELD_VEL, ncomp=3, writable=False)
@@ -438,6 +440,26 @@ class LBMSolverBase:
                 f"Radiation: {self.RADIATION}")
 
     __repr__ = __str__
+
+    def set_mrt_rates(self, rates):
+        """Relaxation rates of the MRT moments (collision="mrt"), in the order of the
+        moment basis (D3Q19: rho, e, eps, jx, qx, jy, qy, jz, qz, 3pxx, 3pixx, pww,
+        piww, pxy, pyz, pxz, mx, my, mz; D2Q9: rho, e, eps, jx, qx, jy, qy, pxx, pxy).
+        An entry <= 0 relaxes that moment with 1/tau, the BGK rate; the shear moments
+        should normally stay at 1/tau so the viscosity follows the viscosity model."""
+        rates = [float(v) for v in rates]
+        if len(rates) > 19:
+            raise ValueError("at most 19 MRT rates")
+        self.mrt_rates = rates + [0.0] * (19 - len(rates))
+
+    def set_mrt_standard_rates(self):
+        """Non-hydrodynamic rates of d'Humieres et al. (2002) for D3Q19 (s_e 1.19, s_eps 1.4,
+        s_q 1.2, s_pi 1.4, s_m 1.98) or Lallemand & Luo (2000) for D2Q9 (s_e 1.64,
+        s_eps 1.54, s_q 1.9); density, momentum and shear moments relax with 1/tau."""
+        if self._dim == 3:
+            self.set_mrt_rates([0, 1.19, 1.4, 0, 1.2, 0, 1.2, 0, 1.2, 0, 1.4, 0, 1.4, 0, 0, 0, 1.98, 1.98, 1.98])
+        else:
+            self.set_mrt_rates([0, 1.64, 1.54, 0, 1.9, 0, 1.9, 0, 0])
 
     def __call__(self):
         # LBM3D/_core.py:111-112
@@ -887,6 +909,8 @@ class LBMSolverBase:
         if self.max_velocity > 0:
             p.check_divergence = 1
             p.max_velocity = float(self.max_velocity)
+        for r, v in enumerate(self.mrt_rates):
+            p.mrt_rates[r] = float(v)
         return p
 
     def _push_params(self):

@@ -105,12 +105,13 @@ SPECIES_3D = (("volatiles", False, 30.0), ("tar", False, 110.0), ("gas", False, 
               ("biomass", True, 162.0), ("char", True, 12.0))
 
 
-def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", collision="bgk"):
-    """3D coupled case: D3Q19 flow, D3Q7 fluid/solid temperature and 4 species
-    (volatiles, tar, gas, inert) plus FIX biomass/char; gray spheres (radius 6-12 lu,
-    solid fraction 0.5 inside) with biomass density 1.0; three competing pyrolysis
-    reactions biomass -> tar, biomass -> gas, biomass -> 0.35 char + 0.65 volatiles;
-    heated inlet T = 1 (800 K)."""
+def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", collision="mrt"):
+    """3D coupled case: D3Q19 flow (MRT with the d'Humieres 2002 non-hydrodynamic rates
+    by default, or BGK), D3Q7 fluid/solid temperature and 4 species (volatiles, tar,
+    gas, inert) plus FIX biomass/char; gray spheres (radius 6-12 lu, solid fraction
+    0.5 inside) with biomass density 1.0; three competing pyrolysis reactions
+    biomass -> tar, biomass -> gas, biomass -> 0.35 char + 0.65 volatiles; heated
+    inlet T = 1 (800 K)."""
     from .solver import LBM3DSolver
     dx = dt = 1e-4
     lb = LBM3DSolver(nx * dx, ny * dx, nz * dx, dx=dx, dt=dt, name="coupled3d", isThermal=True, isChemical=True,
@@ -120,6 +121,8 @@ def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", col
     lb.source_term_model = SOURCE_TERM.MICRO
     lb.force_term_model = FORCE_TERM.GUO
     lb.set_viscosity(1e-5)
+    if collision == "mrt":
+        lb.set_mrt_standard_rates()
     solid, mask = gray_packing((nx, ny, nz), 6.0, 12.0, 0.5, seed, 0.5)
     lb.add_solid(solid)
     lb.set_poro_Darcy(mask * 1.0, 2e6)

@@ -71,6 +71,8 @@ struct lbm_handle {
     int LR[19];
     int es[7][3];
     real ws[7];
+    /* MRT (LBM_COLL_MRT): moment basis M[r][k] and its numerical inverse */
+    double M[19][19], Minv[19][19];
     /* flow */
     real *f, *F, *rho, *v, *solid, *rhos, *cd, *cf;
     /* thermal */
@@ -166,6 +168,85 @@ static void static_init(lbm_handle* h) {
             h->ws[k] = k == 0 ? (real)(1.0 / 3.0) : (real)(1.0 / 6.0);
         }
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* MRT moment basis (not in the reference, which is BGK only).           */
+/* D3Q19: d'Humieres, Ginzburg, Krafczyk, Lallemand, Luo, Phil. Trans.    */
+/* R. Soc. A 360 (2002) 437, eq. (2.1)-(2.3) ordering rho, e, eps, jx, qx, */
+/* jy, qy, jz, qz, 3pxx, 3pixx, pww, piww, pxy, pyz, pxz, mx, my, mz.      */
+/* D2Q9: Lallemand & Luo, Phys. Rev. E 61 (2000) 6546: rho, e, eps, jx,   */
+/* qx, jy, qy, pxx, pxy.  Rows are polynomials of the lattice velocity,   */
+/* so the basis follows the lattice ordering above; the inverse is        */
+/* computed by Gauss-Jordan elimination (no use of orthogonality).        */
+/* ------------------------------------------------------------------ */
+static double mrt_row(int dim, int r, const int* e) {
+    const double x = e[0], y = e[1], z = e[2];
+    const double e2 = x * x + y * y + z * z;
+    if (dim == 2) {
+        switch (r) {
+            case 0: return 1;
+            case 1: return -4 + 3 * e2;
+            case 2: return 4 - 10.5 * e2 + 4.5 * e2 * e2;
+            case 3: return x;
+            case 4: return (-5 + 3 * e2) * x;
+            case 5: return y;
+            case 6: return (-5 + 3 * e2) * y;
+            case 7: return x * x - y * y;
+            default: return x * y;
+        }
+    }
+    switch (r) {
+        case 0: return 1;
+        case 1: return 19 * e2 - 30;
+        case 2: return (21 * e2 * e2 - 53 * e2 + 24) / 2;
+        case 3: return x;
+        case 4: return (5 * e2 - 9) * x;
+        case 5: return y;
+        case 6: return (5 * e2 - 9) * y;
+        case 7: return z;
+        case 8: return (5 * e2 - 9) * z;
+        case 9: return 3 * x * x - e2;
+        case 10: return (3 * e2 - 5) * (3 * x * x - e2);
+        case 11: return y * y - z * z;
+        case 12: return (3 * e2 - 5) * (y * y - z * z);
+        case 13: return x * y;
+        case 14: return y * z;
+        case 15: return x * z;
+        case 16: return (y * y - z * z) * x;
+        case 17: return (z * z - x * x) * y;
+        default: return (x * x - y * y) * z;
+    }
+}
+static int mrt_init(lbm_handle* h) {
+    const int Q = h->Q;
+    double A[19][38];
+    for (int r = 0; r < Q; ++r)
+        for (int k = 0; k < Q; ++k) h->M[r][k] = mrt_row(h->d.dim, r, h->e[k]);
+    for (int r = 0; r < Q; ++r)
+        for (int c = 0; c < 2 * Q; ++c) A[r][c] = c < Q ? h->M[r][c] : (c - Q == r ? 1.0 : 0.0);
+    for (int c = 0; c < Q; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < Q; ++r)
+            if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        if (fabs(A[piv][c]) < 1e-12) return -1;
+        if (piv != c)
+            for (int j = 0; j < 2 * Q; ++j) {
+                double t = A[c][j];
+                A[c][j] = A[piv][j];
+                A[piv][j] = t;
+            }
+        const double d = A[c][c];
+        for (int j = 0; j < 2 * Q; ++j) A[c][j] /= d;
+        for (int r = 0; r < Q; ++r)
+            if (r != c) {
+                const double f = A[r][c];
+                for (int j = 0; j < 2 * Q; ++j) A[r][j] -= f * A[c][j];
+            }
+    }
+    for (int k = 0; k < Q; ++k)
+        for (int r = 0; r < Q; ++r) h->Minv[k][r] = A[k][Q + r];
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -298,6 +379,54 @@ static real force_guo(const lbm_handle* h, int s, size_t i) {
         term = ((real)1.0 - (real)1.0 / (real)2.0 / tau) * rho * h->w[s] *
                ((real)3.0 * dotr(em, F) + (real)9.0 * eu * eF / ee);
     return term;
+}
+
+/* Guo source of direction s without the (1 - 1/(2 tau)) factor: the moment-space
+ * form multiplies each moment by (1 - s_r/2) instead */
+static real guo_shape(const lbm_handle* h, int s, size_t i) {
+    real rho = h->rho[i];
+    const real* u = &h->v[3 * i];
+    real F[3];
+    darcy_force(h, i, F);
+    real eps = (real)1.0 - h->solid[i];
+    real ee = (h->p.eos == LBM_EOS_INCOMPRESSIBLE && h->d.dim == 2) ? eps : eps + (real)1e-12;
+    real em[3];
+    for (int a = 0; a < 3; ++a) em[a] = (real)h->e[s][a] - u[a] / ee;
+    real eu = dot3(h->e[s], u), eF = dot3(h->e[s], F);
+    real shape = h->w[s] * ((real)3.0 * dotr(em, F) + (real)9.0 * eu * eF / ee);
+    return h->p.eos == LBM_EOS_INCOMPRESSIBLE ? shape : rho * shape;
+}
+
+/* MRT collision of the flow populations of cell i (LBM_COLL_MRT):
+ *   m = M f, m_eq = M f_eq, m_F = M G_shape,
+ *   m*_r = m_r - s_r (m_r - m_eq_r) + (1 - s_r / 2) m_F_r,  f* = M^-1 m*,
+ * with s_r = mrt_rates[r], or 1/tau where that entry is <= 0.  All s_r = 1/tau
+ * gives the reference's BGK + Guo update. */
+static void mrt_collide(const lbm_handle* h, size_t i, real* fpost) {
+    const int Q = h->Q;
+    const real* f = &h->f[i * Q];
+    const real itau = 1 / tau_flow(h, i);
+    const int guo = h->p.force_term == LBM_FORCE_GUO;
+    real fe[19], G[19], mpost[19];
+    for (int k = 0; k < Q; ++k) {
+        fe[k] = feq(h, k, i);
+        G[k] = guo ? guo_shape(h, k, i) : (real)0;
+    }
+    for (int r = 0; r < Q; ++r) {
+        real m = 0, meq = 0, mF = 0;
+        for (int k = 0; k < Q; ++k) {
+            m += (real)h->M[r][k] * f[k];
+            meq += (real)h->M[r][k] * fe[k];
+            mF += (real)h->M[r][k] * G[k];
+        }
+        const real sr = h->p.mrt_rates[r] > 0 ? (real)h->p.mrt_rates[r] : itau;
+        mpost[r] = m - sr * (m - meq) + (1 - sr / 2) * mF;
+    }
+    for (int k = 0; k < Q; ++k) {
+        real v = 0;
+        for (int r = 0; r < Q; ++r) v += (real)h->Minv[k][r] * mpost[r];
+        fpost[k] = v;
+    }
 }
 
 /* ---- scalar equilibria ---- */
@@ -545,9 +674,12 @@ static void collision_source_streaming(lbm_handle* h) {
         int x, y, z;
         COORD(h, i, &x, &y, &z);
         if (h->solid[i] < 1) {
+            const int mrt = h->d.collision == LBM_COLL_MRT;
+            real fpost[19];
+            if (mrt) mrt_collide(h, i, fpost);
             for (int k = 0; k < Q; ++k) {
                 real* fi = &h->f[i * Q];
-                real f = fi[k] - 1 / tau_flow(h, i) * (fi[k] - feq(h, k, i));
+                real f = mrt ? fpost[k] : fi[k] - 1 / tau_flow(h, i) * (fi[k] - feq(h, k, i));
                 fi[k] = f;
                 if (k < Qs) {
                     if (therm) {
@@ -565,7 +697,7 @@ static void collision_source_streaming(lbm_handle* h) {
                                 g[k] = g[k] - 1 / scalar_tau(h, sp_coefdiff(h, j, i)) * (g[k] - geq_sp(h, k, h->sp[j].S[i], i));
                             }
                 }
-                if (h->p.force_term == LBM_FORCE_GUO) fi[k] += force_guo(h, k, i);
+                if (h->p.force_term == LBM_FORCE_GUO && !mrt) fi[k] += force_guo(h, k, i);
                 if (therm && k < Qs) {
                     h->TF.g[i * Qs + k] += geq_tf(h, k, h->TF.dS[i], i);
                     if (h->solid[i] > 0.0) h->TS.g[i * Qs + k] += geq_ts(h, k, h->TS.dS[i]);
@@ -861,6 +993,10 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out) {
     h->nz = desc->nz;
     h->N = (size_t)h->nx * h->ny * h->nz;
     static_init(h);
+    if (mrt_init(h)) {
+        free(h);
+        return LBM_ERR_INVALID;
+    }
     size_t N = h->N;
     int Q = h->Q, Qs = h->Qs;
     h->f = zalloc(N * Q);
@@ -921,8 +1057,8 @@ int lbm_set_params(lbm_handle* h, const lbm_params* p) {
     if (!h || !p) return LBM_ERR_INVALID;
     if (p->n_reactions < 0 || p->n_reactions > LBM_MAX_REACTIONS)
         return set_err(h, LBM_ERR_INVALID, "n_reactions out of range");
-    if (h->d.collision != LBM_COLL_BGK)
-        return set_err(h, LBM_ERR_UNSUPPORTED, "the oracle restates the reference BGK collision only");
+    if (h->d.collision != LBM_COLL_BGK && h->d.collision != LBM_COLL_MRT)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "unknown collision model %d", h->d.collision);
     h->p = *p;
     h->params_set = 1;
     return LBM_OK;

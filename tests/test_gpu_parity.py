@@ -22,6 +22,14 @@ CASES = {
     "3d_chem_mixed_solid": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, solidmode="mixed"),
     "3d_chem_df": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, poro_model="df"),
     "3d_spherical": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, poro_model="spherical"),
+    # multiple-relaxation-time flow collision (standard non-hydrodynamic rates)
+    "2d_mrt_thermal_mixed_solid": dict(dim=2, n=(24, 16), therm=True, poro=True, solidmode="mixed_interior",
+                                       collision="mrt"),
+    "3d_mrt_chem_mixed_solid": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, solidmode="mixed_interior",
+                                    collision="mrt"),
+    "3d_mrt_chem": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, collision="mrt"),
+    "3d_mrt_fastface_chem": dict(dim=3, n=(9, 8, 64), therm=True, poro=True, chem=True, collision="mrt"),
+    "3d_mrt_spherical": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, poro_model="spherical", collision="mrt"),
     # two fluid species and one FIX species in the kernel built for four and two
     "3d_chem_partial_species": dict(dim=3, n=(12, 10, 9), therm=True, poro=True, chem=True, species="partial"),
     "2d_chem_partial_species": dict(dim=2, n=(24, 16), therm=True, poro=True, chem=True, species="partial"),
@@ -88,15 +96,16 @@ def test_reductions_match_checker(hip, oracle64):
     assert A.get_max_v() == pytest.approx(B.get_max_v(), rel=1e-12)
 
 
+@pytest.mark.parametrize("collision", ["mrt", "bgk"])
 @pytest.mark.parametrize("nz", [16, 64])
-def test_coupled_workload_matches_checker(nz, hip, oracle64):
+def test_coupled_workload_matches_checker(nz, collision, hip, oracle64):
     """The 3D coupled pyrolysis setup runs the specialised kernel (compile-time model
     switches, flow lattice finished before the scalar lattices); nz = 64 also runs the
     z-face boundary inside the bulk kernel."""
     from lbm_porouschem_amd import workloads
-    print(f"[parity coupled3d] 12x10x{nz} fp64", flush=True)
-    A = workloads.coupled3d(lib=hip, nx=12, ny=10, nz=nz, precision="fp64")
-    B = workloads.coupled3d(lib=oracle64, nx=12, ny=10, nz=nz, precision="fp64")
+    print(f"[parity coupled3d] 12x10x{nz} fp64 {collision}", flush=True)
+    A = workloads.coupled3d(lib=hip, nx=12, ny=10, nz=nz, precision="fp64", collision=collision)
+    B = workloads.coupled3d(lib=oracle64, nx=12, ny=10, nz=nz, precision="fp64", collision=collision)
     A.init_simulation()
     B.init_simulation()
     for nst in (1, 2, 9):
