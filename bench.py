@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "fp64"])
     ap.add_argument("--collision", default="mrt", choices=["mrt", "bgk"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -148,8 +148,11 @@ def main():
 
     N = n ** 3
     nface = N - (n - 2) ** 3
-    B = workloads.COUPLED3D_BYTES_FP32 if args.precision == "fp32" else workloads.COUPLED3D_BYTES_FP32 * 2
-    B_gather = B // 2 + 20  # face cells: populations read + aux reads only in the bulk kernel
+    B = workloads.COUPLED3D_BYTES[args.precision]
+    aux = 8 if args.precision == "fp64" else 4
+    # face cells: the bulk kernel only reads their populations and their 6 aux values
+    # (3 read-only + 3 read-write); their collision and writes happen in k_face
+    B_gather = (B - aux * 9) // 2 + aux * 6
     bulk_bytes = B * (N - nface) + B_gather * nface
     bulk_s = ms_bulk / n_bulk / 1e3
     achieved = bulk_bytes / bulk_s / 1e9
@@ -166,7 +169,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "dtype": {"fp32": "f32", "fp64": "f64", "fp16": "f32 (f16 population storage)"}[args.precision],
         "data": "synthetic (seeded gray sphere packing, seeded Arrhenius parameters)",
         "config": {
             "workload": f"coupled3d {n}x{n}x{n} {args.precision} {args.collision.upper()}: D3Q19 flow + D3Q7 TF + "

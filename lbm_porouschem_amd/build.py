@@ -31,6 +31,7 @@ UNITS = [
     ("lbm_abi.hip", []),
     ("lbm_f64.hip", ["-ffp-contract=off"]),
     ("lbm_f32.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("lbm_f16.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
 ]
 
 

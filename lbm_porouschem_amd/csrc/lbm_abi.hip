@@ -15,6 +15,7 @@ Impl* make_impl(lbm_handle* h) {
         for (int j = 0; j < d.n_species; ++j) (d.species_fix[j] ? nfix : nf)++;
     if (d.precision == LBM_PREC_FP64) return make_impl_f64(h, nf, nfix);
     if (d.precision == LBM_PREC_FP32) return make_impl_f32(h, nf, nfix);
+    if (d.precision == LBM_PREC_FP16) return make_impl_f16(h, nf, nfix);
     return nullptr;
 }
 

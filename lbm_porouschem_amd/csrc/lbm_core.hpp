@@ -173,6 +173,7 @@ template <class Real> struct DevParams {
     int nf, nfix;             // species counts in use (<= template maxima)
     int eos, visc_model, force_guo, poro_model, rad_model, bc_nee;
     int coll;                 // COLL_BGK / COLL_MRT
+    Real pref[3 + MAXS];      // FP16 storage: reference value per lattice (flow, TF, TS, species)
     int mrt_tau_mask;         // bit r: moment r relaxes with 1/tau
     Real mrt_s[19];           // fixed MRT rates of the other moments
     int bc_v[6], bc_rho[6];
@@ -221,10 +222,26 @@ template <class Real, class Store> struct Bufs {
 // ---------------------------------------------------------------------------
 // population storage codecs
 // ---------------------------------------------------------------------------
+// Population storage: a lattice stored in the arithmetic type holds f itself; an FP16
+// lattice holds the deviation f - w_k * ref from the weighted reference value of its
+// lattice (rho0 for the flow lattice, the initial mean of T or S for a scalar one),
+// so the 11-bit significand spends its precision on the part of f that varies.  The
+// deviation is stored times 2^12 (exact): values down to ~1.5e-8 (a trace species
+// that starts at zero) stay in FP16's normal range, and |deviation| up to 16 fits.
 template <class Store, class Real> struct Codec {
+    static constexpr bool offset = false;
     LBM_DEV static Real load(const Store* p, Real) { return (Real)*p; }
     LBM_DEV static void store(Store* p, Real v, Real) { *p = (Store)v; }
 };
+template <> struct Codec<_Float16, float> {
+    static constexpr bool offset = true;
+    LBM_DEV static float load(const _Float16* p, float off) { return (float)*p * (1.0f / 4096.0f) + off; }
+    LBM_DEV static void store(_Float16* p, float v, float off) { *p = (_Float16)((v - off) * 4096.0f); }
+};
+// lattice weight of direction k of a QN-velocity set (19/9 flow, 7/5 scalar)
+template <int QN> LBM_DEV constexpr double lat_w(int k) {
+    return QN == 19 ? Lat<3>::w(k) : (QN == 9 ? Lat<2>::w(k) : (QN == 7 ? Lat<3>::ws(k) : Lat<2>::ws(k)));
+}
 
 // ---------------------------------------------------------------------------
 // geometry helpers
@@ -1099,11 +1116,19 @@ template <int D> LBM_DEV long long nb_of(const Geo& g, int k) {
 
 // gather the incoming populations of one lattice (QN directions, slots base..base+QN-1)
 // bitbase: mask bit of direction 0 for this lattice's "evolves at neighbour" test
+// reference value of the lattice whose first slot is `base` (flow lattice at slot 0,
+// scalar lattices of QN slots each from slot_tf on)
+template <int QN, class Real> LBM_DEV Real lattice_ref(const DevParams<Real>& P, int base) {
+    return base == 0 ? P.pref[0] : P.pref[1 + (base - P.slot_tf) / QN];
+}
+
 template <int D, int QN, int PAR, class Real, class Store>
-LBM_DEV void gather(const Store* __restrict__ pops, long long N, int base, const Geo& g, uint32_t mask, int bitbase,
-                    Real* out) {
+LBM_DEV void gather(const DevParams<Real>& P, const Store* __restrict__ pops, int base, const Geo& g, uint32_t mask,
+                    int bitbase, Real* out) {
     using L = Lat<D>;
     using O = typename OffT<Store>::type;
+    const long long N = P.N;
+    const Real ref = Codec<Store, Real>::offset ? lattice_ref<QN>(P, base) : (Real)0;
     const Store* lb = pops + (long long)base * N;
     const O c = (O)g.c, n = (O)N;
 #pragma unroll
@@ -1117,15 +1142,17 @@ LBM_DEV void gather(const Store* __restrict__ pops, long long N, int base, const
             const O own = (O)k * n + c;
             e = ((mask >> (bitbase + kr)) & 1u) ? far : own;
         }
-        out[k] = Codec<Store, Real>::load(at(lb, e), (Real)0);
+        out[k] = Codec<Store, Real>::load(at(lb, e), (Real)lat_w<QN>(k) * ref);
     }
 }
 // scatter post-collision populations with the same parity rule
 template <int D, int QN, int PAR, class Real, class Store>
-LBM_DEV void scatter(Store* __restrict__ pops, long long N, int base, const Geo& g, uint32_t mask, int bitbase,
-                     const Real* in) {
+LBM_DEV void scatter(const DevParams<Real>& P, Store* __restrict__ pops, int base, const Geo& g, uint32_t mask,
+                     int bitbase, const Real* in) {
     using L = Lat<D>;
     using O = typename OffT<Store>::type;
+    const long long N = P.N;
+    const Real ref = Codec<Store, Real>::offset ? lattice_ref<QN>(P, base) : (Real)0;
     Store* lb = pops + (long long)base * N;
     const O c = (O)g.c, n = (O)N;
 #pragma unroll
@@ -1139,23 +1166,27 @@ LBM_DEV void scatter(Store* __restrict__ pops, long long N, int base, const Geo&
             const O own = (O)kr * n + c;
             e = ((mask >> (bitbase + k)) & 1u) ? far : own;
         }
-        Codec<Store, Real>::store(at(lb, e), in[k], (Real)0);
+        Codec<Store, Real>::store(at(lb, e), in[k], (Real)lat_w<QN>(k) * ref);
     }
 }
 // fixed slots (lattice not evolved at this cell): slot k holds direction k
 template <int QN, class Real, class Store>
-LBM_DEV void load_fixed(const Store* __restrict__ pops, long long N, int base, long long c, Real* out) {
+LBM_DEV void load_fixed(const DevParams<Real>& P, const Store* __restrict__ pops, int base, long long c, Real* out) {
     using O = typename OffT<Store>::type;
+    const long long N = P.N;
+    const Real ref = Codec<Store, Real>::offset ? lattice_ref<QN>(P, base) : (Real)0;
     const Store* lb = pops + (long long)base * N;
 #pragma unroll
-    for (int k = 0; k < QN; ++k) out[k] = Codec<Store, Real>::load(at(lb, (O)k * (O)N + (O)c), (Real)0);
+    for (int k = 0; k < QN; ++k) out[k] = Codec<Store, Real>::load(at(lb, (O)k * (O)N + (O)c), (Real)lat_w<QN>(k) * ref);
 }
 template <int QN, class Real, class Store>
-LBM_DEV void store_fixed(Store* __restrict__ pops, long long N, int base, long long c, const Real* in) {
+LBM_DEV void store_fixed(const DevParams<Real>& P, Store* __restrict__ pops, int base, long long c, const Real* in) {
     using O = typename OffT<Store>::type;
+    const long long N = P.N;
+    const Real ref = Codec<Store, Real>::offset ? lattice_ref<QN>(P, base) : (Real)0;
     Store* lb = pops + (long long)base * N;
 #pragma unroll
-    for (int k = 0; k < QN; ++k) Codec<Store, Real>::store(at(lb, (O)k * (O)N + (O)c), in[k], (Real)0);
+    for (int k = 0; k < QN; ++k) Codec<Store, Real>::store(at(lb, (O)k * (O)N + (O)c), in[k], (Real)lat_w<QN>(k) * ref);
 }
 
 // side buffer (shell cells), same slot numbering as pops
@@ -1295,15 +1326,15 @@ template <int D, int PAR, class Real, class Store, int NF, int NX, class M = Mod
 LBM_DEV void scatter_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, const Geo& g, uint32_t mask,
                          const CellPops<Real, D, NF, NX>& p, bool pts) {
     using L = Lat<D>;
-    scatter<D, L::Q, PAR>(B.pops, P.N, 0, g, mask, 0, p.f);
+    scatter<D, L::Q, PAR>(P, B.pops, 0, g, mask, 0, p.f);
     if (LBM_MV(thermal, P.thermal)) {
-        scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_tf, g, mask, 0, p.gT);
-        if (pts) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_ts, g, mask, TS_BIT, p.gS);
-        else store_fixed<L::QS>(B.pops, P.N, P.slot_ts, g.c, p.gS);
+        scatter<D, L::QS, PAR>(P, B.pops, P.slot_tf, g, mask, 0, p.gT);
+        if (pts) scatter<D, L::QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
+        else store_fixed<L::QS>(P, B.pops, P.slot_ts, g.c, p.gS);
     }
 #pragma unroll
     for (int j = 0; j < NF; ++j)
-        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) scatter<D, L::QS, PAR>(B.pops, P.N, P.slot_sp + j * L::QS, g, mask, 0, p.gP[j]);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) scatter<D, L::QS, PAR>(P, B.pops, P.slot_sp + j * L::QS, g, mask, 0, p.gP[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1513,14 +1544,14 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     if (!pflow) {
         // fully solid cell: flow, TF and species are frozen; TS relaxes from its own populations
         if (shell) {
-            load_fixed<L::Q>(B.pops, N, 0, c, p.f);
+            load_fixed<L::Q>(P, B.pops, 0, c, p.f);
             if (LBM_MV(thermal, P.thermal)) {
-                load_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
-                gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+                load_fixed<QS>(P, B.pops, P.slot_tf, c, p.gT);
+                gather<D, QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
             }
 #pragma unroll
             for (int j = 0; j < NF; ++j)
-                if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) load_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+                if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) load_fixed<QS>(P, B.pops, P.slot_sp + j * QS, c, p.gP[j]);
             side_put_all<Real, Store, D, NF, NX, M>(P, B, si, p);
         }
         if (face || !LBM_MV(thermal, P.thermal)) return;
@@ -1535,7 +1566,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
         collide_emitter<Real, D, NF, NX, M>(P, ge, m);
 #pragma unroll
         for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = ge[k];
-        scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, ge);
+        scatter<D, QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, ge);
         return;
     }
 
@@ -1545,7 +1576,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
         // one group of populations is live at a time
         load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
         Real f[L::Q];
-        gather<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, f);
+        gather<D, L::Q, PAR>(P, B.pops, 0, g, mask, 0, f);
         {
             Real Sdummy[NF > 0 ? NF : 1];
 #pragma unroll
@@ -1573,13 +1604,13 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
             mb.uy = uy_b;
             mb.uz = uz_b;
             collide_flow<Real, D, NF, NX, M>(P, f, mb);
-            scatter<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, f);
+            scatter<D, L::Q, PAR>(P, B.pops, 0, g, mask, 0, f);
         }
         // ---- scalar lattices ----
         if (LBM_MV(thermal, P.thermal)) {
-            gather<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
-            if (pts) gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
-            else if (shell || write_macro || zfast) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+            gather<D, QS, PAR>(P, B.pops, P.slot_tf, g, mask, 0, p.gT);
+            if (pts) gather<D, QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
+            else if (shell || write_macro || zfast) load_fixed<QS>(P, B.pops, P.slot_ts, c, p.gS);
             else {
 #pragma unroll
                 for (int k = 0; k < QS; ++k) p.gS[k] = (Real)0;
@@ -1588,7 +1619,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
 #pragma unroll
         for (int j = 0; j < NF; ++j)
             if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf))
-                gather<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+                gather<D, QS, PAR>(P, B.pops, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
 #pragma unroll
         for (int j = 0; j < NF + NX; ++j) {
             m.S[j] = (Real)0;
@@ -1629,23 +1660,23 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
         for (int j = 0; j < NX; ++j)
             if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
         if (LBM_MV(thermal, P.thermal)) {
-            scatter<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
-            if (pts) scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
-            else store_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+            scatter<D, QS, PAR>(P, B.pops, P.slot_tf, g, mask, 0, p.gT);
+            if (pts) scatter<D, QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
+            else store_fixed<QS>(P, B.pops, P.slot_ts, c, p.gS);
         }
 #pragma unroll
         for (int j = 0; j < NF; ++j)
             if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf))
-                scatter<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+                scatter<D, QS, PAR>(P, B.pops, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
         return;
     }
 
     load_static<Real, Store, NF, NX, M>(P, B, c, mask, m);
-    gather<D, L::Q, PAR>(B.pops, N, 0, g, mask, 0, p.f);
+    gather<D, L::Q, PAR>(P, B.pops, 0, g, mask, 0, p.f);
     if (LBM_MV(thermal, P.thermal)) {
-        gather<D, QS, PAR>(B.pops, N, P.slot_tf, g, mask, 0, p.gT);
-        if (pts) gather<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
-        else if (shell || write_macro || zfast) load_fixed<QS>(B.pops, N, P.slot_ts, c, p.gS);
+        gather<D, QS, PAR>(P, B.pops, P.slot_tf, g, mask, 0, p.gT);
+        if (pts) gather<D, QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
+        else if (shell || write_macro || zfast) load_fixed<QS>(P, B.pops, P.slot_ts, c, p.gS);
         else {
 #pragma unroll
             for (int k = 0; k < QS; ++k) p.gS[k] = (Real)0;
@@ -1653,7 +1684,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
     }
 #pragma unroll
     for (int j = 0; j < NF; ++j)
-        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) gather<D, QS, PAR>(B.pops, N, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
+        if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) gather<D, QS, PAR>(P, B.pops, P.slot_sp + j * QS, g, mask, 0, p.gP[j]);
 #pragma unroll
     for (int j = 0; j < NF + NX; ++j) {
         m.S[j] = (Real)0;
@@ -1813,16 +1844,16 @@ LBM_DEV void face_collide_cell(const DevParams<Real>& P, const Bufs<Real, Store>
     for (int j = 0; j < NX; ++j)
         if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) m.S[NF + j] = B.fixS[(long long)j * N + c];
     if (!pflow) {
-        store_fixed<L::Q>(B.pops, N, 0, c, p.f);
-        if (LBM_MV(thermal, P.thermal)) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+        store_fixed<L::Q>(P, B.pops, 0, c, p.f);
+        if (LBM_MV(thermal, P.thermal)) store_fixed<QS>(P, B.pops, P.slot_tf, c, p.gT);
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) store_fixed<QS>(P, B.pops, P.slot_sp + j * QS, c, p.gP[j]);
         if (LBM_MV(thermal, P.thermal)) {
             collide_emitter<Real, D, NF, NX, M>(P, p.gS, m);
 #pragma unroll
             for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = p.gS[k];
-            scatter<D, QS, PAR>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+            scatter<D, QS, PAR>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
         }
         return;
     }
@@ -1877,11 +1908,11 @@ LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
         B.u[2 * N + c] = m.uz;
     }
     if (!pflow) {
-        store_fixed<Q>(B.pops, N, 0, c, p.f);
-        if (LBM_MV(thermal, P.thermal)) store_fixed<QS>(B.pops, N, P.slot_tf, c, p.gT);
+        store_fixed<Q>(P, B.pops, 0, c, p.f);
+        if (LBM_MV(thermal, P.thermal)) store_fixed<QS>(P, B.pops, P.slot_tf, c, p.gT);
 #pragma unroll
         for (int j = 0; j < NF; ++j)
-            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) store_fixed<QS>(B.pops, N, P.slot_sp + j * QS, c, p.gP[j]);
+            if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) store_fixed<QS>(P, B.pops, P.slot_sp + j * QS, c, p.gP[j]);
 #pragma unroll
         for (int j = 0; j < NX; ++j)
             if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NX : P.nfix)) B.fixS[(long long)j * N + c] = m.S[NF + j];
@@ -1889,7 +1920,7 @@ LBM_DEV void init_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
             collide_emitter<Real, D, NF, NX, M>(P, p.gS, m);
 #pragma unroll
             for (int k = 0; k < QS; ++k) B.emit[(long long)k * N + c] = p.gS[k];
-            scatter<D, QS, 0>(B.pops, N, P.slot_ts, g, mask, TS_BIT, p.gS);
+            scatter<D, QS, 0>(P, B.pops, P.slot_ts, g, mask, TS_BIT, p.gS);
         }
         return;
     }

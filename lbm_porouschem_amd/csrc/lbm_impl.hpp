@@ -562,6 +562,8 @@ struct ImplT final : Impl {
                       Q_.tf.cap_model == THERMO_CONSTANT && Q_.ts.diff_model == TDIFF_CONSTANT &&
                       Q_.ts.cap_model == THERMO_CONSTANT && sp_const && !Q_.lag_tf && !Q_.lag_sp;
         }
+        // the FP16 reference values are fixed at lbm_init and survive later parameter updates
+        for (int i = 0; i < 3 + MAXS; ++i) Q_.pref[i] = P.pref[i];
         P = Q_;
         HIP_TRY(hipMemcpyAsync(dP, &P, sizeof(P), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
@@ -695,6 +697,12 @@ struct ImplT final : Impl {
                 if (rc) return rc;
             }
         }
+        if (Codec<Store, Real>::offset) {
+            // reference values of the FP16 deviation storage: the initial mean of each
+            // lattice's macroscopic field (rho, TF, TS, species S)
+            int rc = set_lattice_refs();
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL((k_mask<Real, D>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, solid, (uint32_t*)B.mask, N,
                            nx, ny, nz);
         HIP_TRY(hipGetLastError());
@@ -751,6 +759,38 @@ struct ImplT final : Impl {
         hipLaunchKernelGGL((k_init<Real, Store, D, NF, NX>), dim3(grid_for(N)), dim3(BLOCK), 0, stream, dP, B, inflow,
                            v0);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    int set_lattice_refs() {
+        auto mean_of = [&](const Real* dptr, double& out) -> int {
+            std::vector<Real> t((size_t)N);
+            HIP_TRY(hipMemcpyAsync(t.data(), dptr, N * sizeof(Real), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            double acc = 0;
+            for (long long i = 0; i < N; ++i) acc += (double)t[(size_t)i];
+            out = acc / (double)N;
+            return LBM_OK;
+        };
+        double v = 0;
+        int rc;
+        for (int i = 0; i < 3 + MAXS; ++i) P.pref[i] = (Real)0;
+        if ((rc = mean_of(B.rho, v))) return rc;
+        P.pref[0] = (Real)v;
+        // scalar lattice id = 1 + (slot - slot_tf) / QS, as lattice_ref computes it
+        if (h_->d.thermal) {
+            if ((rc = mean_of(B.TF, v))) return rc;
+            P.pref[1 + (P.slot_tf - P.slot_tf) / QS] = (Real)v;
+            if ((rc = mean_of(B.TS, v))) return rc;
+            P.pref[1 + (P.slot_ts - P.slot_tf) / QS] = (Real)v;
+        }
+        if (h_->d.chemical)
+            for (int j = 0; j < nf; ++j) {
+                if ((rc = mean_of(B.S + (long long)j * N, v))) return rc;
+                P.pref[1 + (P.slot_sp + j * QS - P.slot_tf) / QS] = (Real)v;
+            }
+        HIP_TRY(hipMemcpyAsync(dP, &P, sizeof(P), hipMemcpyHostToDevice, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         return LBM_OK;
     }
@@ -883,6 +923,7 @@ struct ImplT final : Impl {
 // so the FP64 kernels can be compiled without floating-point contraction
 Impl* make_impl_f64(lbm_handle* h, int nf, int nfix);
 Impl* make_impl_f32(lbm_handle* h, int nf, int nfix);
+Impl* make_impl_f16(lbm_handle* h, int nf, int nfix);
 
 template <class Real, class Store, int D>
 Impl* make_for_species(lbm_handle* h, int nf, int nfix) {

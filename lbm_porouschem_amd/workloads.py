@@ -174,4 +174,9 @@ def bytes_per_cell(q_tot, pop_bytes, n_ro, n_rw, aux_bytes):
 
 COUPLED3D_Q = 19 + 7 + 7 + 4 * 7  # 61
 COUPLED3D_BYTES_FP32 = bytes_per_cell(COUPLED3D_Q, 4, 3, 3, 4)  # 524
+COUPLED3D_BYTES = {
+    "fp64": bytes_per_cell(COUPLED3D_Q, 8, 3, 3, 8),  # 1048
+    "fp32": COUPLED3D_BYTES_FP32,                      # 524
+    "fp16": bytes_per_cell(COUPLED3D_Q, 2, 3, 3, 4),  # 280: FP16 populations, FP32 aux
+}
 PLAIN2D_BYTES_FP64 = bytes_per_cell(9 + 5 + 5, 8, 3, 0, 8)  # 328
