@@ -264,6 +264,15 @@ long long lbm_steps_done(const lbm_handle* h);
 int lbm_profile(lbm_handle* h, int enable);
 int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* launches);
 
+/* Snapshot / resume.  The reference's export_snapshot is a stub (LBM3D/_io.py:593-595);
+ * these copy the complete evolving state of an initialised handle (populations, FIX
+ * densities, lagged fields, macroscopic fields, boundary profiles and the step
+ * counter) to and from an opaque host blob.  A blob is valid only for a handle
+ * created from the same descriptor and parameters by the same library. */
+int lbm_state_size(const lbm_handle* h, size_t* bytes);
+int lbm_get_state(lbm_handle* h, void* host, size_t bytes);
+int lbm_set_state(lbm_handle* h, const void* host, size_t bytes);
+
 /* sizeof(lbm_desc) for which = 0, sizeof(lbm_params) for which = 1 (binding self-check). */
 size_t lbm_struct_size(int which);
 

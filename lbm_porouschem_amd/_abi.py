@@ -127,7 +127,7 @@ class DivergenceError(LbmError):
 _SYMBOLS = [
     "lbm_abi_version", "lbm_struct_size", "lbm_create", "lbm_destroy", "lbm_set_params", "lbm_set_field",
     "lbm_get_field", "lbm_init", "lbm_step", "lbm_reduce", "lbm_sync", "lbm_steps_done", "lbm_last_error",
-    "lbm_profile", "lbm_profile_read",
+    "lbm_profile", "lbm_profile_read", "lbm_state_size", "lbm_get_state", "lbm_set_state",
 ]
 
 
@@ -168,6 +168,12 @@ class CLib:
         L.lbm_profile_read.argtypes = [ctypes.c_void_p, _c_int, ctypes.POINTER(_c_double),
                                        ctypes.POINTER(ctypes.c_longlong)]
         L.lbm_profile_read.restype = _c_int
+        L.lbm_state_size.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+        L.lbm_state_size.restype = _c_int
+        L.lbm_get_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.lbm_get_state.restype = _c_int
+        L.lbm_set_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.lbm_set_state.restype = _c_int
         if L.lbm_abi_version() != ABI_VERSION:
             raise LbmError(ERR_INVALID, f"{self.path}: ABI version {L.lbm_abi_version()} != {ABI_VERSION}")
         if L.lbm_struct_size(0) != ctypes.sizeof(LbmDesc) or L.lbm_struct_size(1) != ctypes.sizeof(LbmParams):
@@ -227,6 +233,17 @@ class CLib:
         n = ctypes.c_longlong()
         self._check(h, self.lib.lbm_profile_read(h, kernel, ctypes.byref(ms), ctypes.byref(n)), "lbm_profile_read")
         return ms.value, n.value
+
+    def get_state(self, h):
+        n = ctypes.c_size_t()
+        self._check(h, self.lib.lbm_state_size(h, ctypes.byref(n)), "lbm_state_size")
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(h, self.lib.lbm_get_state(h, buf, n.value), "lbm_get_state")
+        return buf.raw
+
+    def set_state(self, h, blob):
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        self._check(h, self.lib.lbm_set_state(h, buf, len(blob)), "lbm_set_state")
 
     def steps_done(self, h):
         return int(self.lib.lbm_steps_done(h))

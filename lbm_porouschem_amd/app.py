@@ -1,0 +1,380 @@
+"""YAML configuration -> solver -> time loop, the host counterpart of the
+reference's `application(config, logger)` (src/LBM/app.py:16-677) driving the HIP
+solver classes.
+
+The configuration schema is the reference's (examples/*/config.yaml): sections
+basic, module, spaceControl, timeControl, flowProperties, thermalPropertiesFluid /
+thermalPropertiesSolid, radiationProperties, chemicalProperties (type "input"),
+boundaryCondition (left, right, down, up, forward, back), geometry, initialCondition
+and outputControl (log, vtk, snapshot).  Differences:
+  * `basic.arch` is informational: the step always runs on the HIP device;
+  * geometry shapes are voxelised directly (geometry.py) instead of through
+    CadQuery/pyvista; an extra shape `packing` (seeded spheres / disks: rmin, rmax,
+    fraction, seed) generates porous media;
+  * chemistry type "cantera" is not supported (the reference's loader only registers
+    species, LBM3D/_io.py:508-537);
+  * the loop advances in batches between log / export / snapshot events, so queued
+    steps run as one lbm_step(n) call;
+  * event intervals are rounded to the nearest lattice step (the reference truncates
+    interval/dt, so 0.0006/1e-4 = 5.999... becomes 5).
+"""
+import logging
+import os
+import time
+
+import numpy as np
+
+from . import geometry
+from .constant import MOLEMASS
+from .flag import BC, BC_FLOW, FORCE_TERM, RADIATION_MODEL, SOURCE_TERM
+
+SIDES = ["left", "right", "down", "up", "forward", "back"]
+
+
+def _yaml_loader():
+    """SafeLoader that also resolves YAML 1.2 floats such as 2e-5 and 1.0e6 (the
+    reference reads its configurations with ruamel.yaml, a YAML 1.2 parser; PyYAML's
+    YAML 1.1 rules would leave these as strings).  Nothing is executed from the file."""
+    import re
+    import yaml
+
+    class Loader(yaml.SafeLoader):
+        pass
+
+    Loader.add_implicit_resolver(
+        "tag:yaml.org,2002:float",
+        re.compile(r"""^(?:[-+]?(?:[0-9][0-9_]*)\.[0-9_]*(?:[eE][-+]?[0-9]+)?
+                    |[-+]?(?:[0-9][0-9_]*)(?:[eE][-+]?[0-9]+)
+                    |\.[0-9_]+(?:[eE][-+]?[0-9]+)?
+                    |[-+]?\.(?:inf|Inf|INF)
+                    |\.(?:nan|NaN|NAN))$""", re.X),
+        list("-+0123456789."))
+    return Loader
+
+
+def load_config(text_or_file):
+    """Parse a configuration (path, open file or YAML text) safely, YAML 1.2 floats included."""
+    import yaml
+    if isinstance(text_or_file, str) and os.path.exists(text_or_file):
+        with open(text_or_file, "r", encoding="utf8") as f:
+            return yaml.load(f, Loader=_yaml_loader())
+    return yaml.load(text_or_file, Loader=_yaml_loader())
+
+
+def _log(logger):
+    return logger if logger is not None else logging.getLogger("lbm_porouschem_amd")
+
+
+def _as_list(zone):
+    if zone is None or zone == "ALL":
+        return None
+    return [zone] if isinstance(zone, str) else list(zone)
+
+
+def build_geometry(lb, geo_infos, logger):
+    """geometry section -> {name: (occupancy, surface weight)} on the lattice."""
+    out = {}
+    shape = (lb.nx, lb.ny, lb.nz)
+    for name, g in (geo_infos or {}).items():
+        kind = g.get("type")
+        c = list(g.get("center", [0, 0, 0])) + [0, 0, 0]
+        if kind == "sphere":
+            r = g.get("radius", 0.5)
+            r = float(r[0] if isinstance(r, (list, tuple)) else r)
+            vox = geometry.sphere(shape if lb._dim == 3 else shape[:2], lb.dx, c, r).reshape(shape)
+        elif kind == "cylinder":
+            vox = geometry.cylinder(shape, lb.dx, c[:3], float(g.get("radius", 0.5)), float(g.get("height", 1)),
+                                    g.get("axis", [0, 0, 1]))
+        elif kind == "cone":
+            vox = geometry.cone(shape, lb.dx, c[:3], float(g.get("radius", 0.5)), float(g.get("height", 1)),
+                                g.get("axis", [0, 0, 1]))
+        elif kind == "box":
+            size = list(g.get("size", [1, 1, 1])) + [1e30]
+            vox = geometry.box(shape if lb._dim == 3 else shape[:2], lb.dx, c, size).reshape(shape)
+        elif kind == "stl":
+            vox, _ = lb.load_stl(g["path"], g.get("scale", [1, 1, 1]), g.get("translate", [0, 0, 0]),
+                                 g.get("rotate", [0, 0, 0]), logger=logger)
+        elif kind == "packing":
+            vox = geometry.packing(shape if lb._dim == 3 else shape[:2], lb.dx, float(g["rmin"]), float(g["rmax"]),
+                                   float(g.get("fraction", 0.5)), int(g.get("seed", 0))).reshape(shape)
+        else:
+            logger.warning(f"Geometry {kind} not valid.")
+            continue
+        if np.any(vox):
+            out[name] = (vox, geometry.surface_weight(vox))
+    return out
+
+
+def _zone_sum(geo, zone, idx):
+    names = _as_list(zone)
+    keys = list(geo) if names is None else names
+    s = None
+    for k in keys:
+        a = geo[k][idx]
+        s = a.copy() if s is None else s + a
+    return s
+
+
+def configure(config, logger=None, lib=None, precision="fp32", collision="bgk"):
+    """Build and initialise-ready solver from a configuration dict (app.py:16-560)."""
+    logger = _log(logger)
+    basic = config["basic"]
+    dim = basic.get("dimension")
+    if dim not in (2, 3):
+        raise ValueError("basic.dimension must be 2 or 3")
+    if basic.get("arch") is not None:
+        logger.info(f"basic.arch = {basic.get('arch')}: the step runs on the HIP device")
+    from .solver import LBM2DSolver, LBM3DSolver
+    cls = LBM2DSolver if dim == 2 else LBM3DSolver
+    geo = config["spaceControl"]["geometry"]
+    x, y = geo[0], geo[1]
+    z = geo[2] if dim == 3 else 1
+    dx = config["spaceControl"]["DX"]
+    dt = config["timeControl"]["DT"]
+    mod = config.get("module", {})
+    isThermal, isPoro = bool(mod.get("temperature")), bool(mod.get("porous_media"))
+    isChemical, isRadiation = bool(mod.get("chemistry")), bool(mod.get("radiation"))
+    lb = cls(x, y, z, dx, dt, basic.get("name", "LBM"), isThermal, isChemical, isPoro, isRadiation,
+             precision=precision, collision=collision, lib=lib)
+    lb.source_term_model = SOURCE_TERM.MICRO
+    lb.force_term_model = FORCE_TERM.GUO
+
+    visc = (config.get("flowProperties") or {}).get("viscosity") or {}
+    vt = visc.get("type")
+    if vt == "constant":
+        lb.set_viscosity(visc.get("value"))
+    elif vt == "sutherland":
+        lb.set_viscosity_sutherland(visc.get("As"), visc.get("Ts"))
+    elif vt == "mixture":
+        lb.set_viscosity_mixture()
+
+    def thermal(section, fluid):
+        tp = config.get(section)
+        if not (isThermal and tp):
+            return
+        pre = "fluid" if fluid else "solid"
+        norm = tp.get("normalize")
+        if norm is not None:
+            getattr(lb, f"set_{pre}_Trange")(norm.get("Trange", [0, 1]))
+        td = tp.get("thermalDiff")
+        if td:
+            t = td.get("type")
+            if t == "constant":
+                getattr(lb, f"set_{pre}_thermal_diff")(td.get("value"))
+            elif t == "Prandtl" and fluid:
+                lb.set_fluid_Prandtl(td.get("Pr"))
+            elif t == "derived":
+                getattr(lb, f"set_{pre}_thermal_diff_derived")()
+        cd = tp.get("conductivity")
+        if cd:
+            t = cd.get("type")
+            if t == "constant":
+                getattr(lb, f"set_{pre}_conductivity")(cd.get("value"))
+            elif t == "polynomial":
+                getattr(lb, f"set_{pre}_conductivity_poly")(cd.get("data"))
+            elif t == "mixture":
+                getattr(lb, f"set_{pre}_conductivity_mixture")()
+            else:
+                logger.warning(f"Conductivity type {t} of {section} not valid.")
+        cp = tp.get("capacity")
+        if cp:
+            t = cp.get("type")
+            if t == "constant":
+                getattr(lb, f"set_{pre}_capacity")(cp.get("value"))
+            elif t == "polynomial":
+                getattr(lb, f"set_{pre}_capacity_poly")(cp.get("data"))
+            elif t == "NASA7":
+                getattr(lb, f"set_{pre}_capacity_NASA7")(cp.get("Trange"), cp.get("data"))
+            elif t == "mixture":
+                getattr(lb, f"set_{pre}_capacity_mixture")()
+            else:
+                logger.warning(f"Capacity type {t} of {section} not valid.")
+
+    thermal("thermalPropertiesFluid", True)
+    thermal("thermalPropertiesSolid", False)
+
+    rad = config.get("radiationProperties")
+    if isRadiation and rad and rad.get("type") == "mean_temp":
+        lb.set_radiation(RADIATION_MODEL.SURFACE_UNIFORM, rad.get("Tambient", 300))
+
+    chem = config.get("chemicalProperties")
+    if isChemical and chem:
+        ct = chem.get("type")
+        if ct == "input":
+            for name, sp in (chem.get("species") or {}).items():
+                comp = sp.get("composition") or {}
+                mm = sum(v * MOLEMASS[k.upper()] for k, v in comp.items())
+                state = sp.get("state", "fluid")
+                lb.set_specie(name, state == "solid", mm if mm else 1.0)
+                if state == "fluid":
+                    vp = sp.get("viscosity")
+                    if vp and vp.get("type") == "constant":
+                        lb.set_specie_viscosity(name, vp.get("value", 1e-5))
+                    elif vp and vp.get("type") == "sutherland":
+                        lb.set_specie_viscosity_sutherland(name, [vp.get("As", 1e-6), vp.get("Ts", 70)])
+                    dp = sp.get("diffusitivity")
+                    if dp and dp.get("type") == "constant":
+                        lb.set_specie_diff(name, dp.get("value"))
+                    elif dp and dp.get("type") == "Schmidt":
+                        lb.set_specie_diff_Schmidt(name, dp.get("Sc"))
+                if isThermal:
+                    tdp = sp.get("thermodynamic")
+                    if tdp and tdp.get("type") == "constant":
+                        lb.set_specie_capacity(name, tdp.get("capacity"))
+                        lb.set_specie_enthalpy(name, tdp.get("enthalpy"))
+                    elif tdp and tdp.get("type") == "NASA7":
+                        lb.set_specie_NASA7(name, tdp.get("Trange"), tdp.get("data"))
+                    cond = sp.get("conductivity")
+                    if cond and cond.get("type") == "constant":
+                        lb.set_specie_conductivity(name, cond.get("value"))
+                    elif cond and cond.get("type") == "polynomial":
+                        lb.set_specie_conductivity_poly(name, cond.get("data"))
+            for r in chem.get("reactions") or []:
+                rc = r["rate-constant"]
+                lb.add_reaction(r["equation"], rc["A"], rc["Ea"], rc.get("b", 0))
+        else:
+            raise ValueError(f"chemicalProperties.type {ct!r} is not supported (use 'input')")
+
+    for i in range(2 * dim):
+        bc = config["boundaryCondition"][SIDES[i]]
+        fl = bc["flow"]
+        t = fl["type"]
+        if t == "inlet":
+            lb.set_BC(i, BC_FLOW.inlet)
+            lb.set_v_BC_value(i, fl["velocity"])
+            if fl.get("rho"):
+                lb.set_rho_BC_value(i, fl["rho"])
+        elif t == "outlet":
+            lb.set_BC(i, BC_FLOW.outlet)
+            lb.set_rho_BC_value(i, fl["rho"])
+        elif t == "inlet_flowrate":
+            lb.set_BC(i, BC_FLOW.inlet_flow)
+            lb.set_flow_BC_value(i, fl["flowrate"])
+            if fl.get("rho", 1):
+                lb.set_rho_BC_value(i, fl.get("rho", 1))
+        elif t == "wall":
+            lb.set_BC(i, BC_FLOW.wall)
+        if isThermal:
+            for key, setter, setv in (("thermalFluid", lb.set_TF_BC, lb.set_TF_BC_value),
+                                      ("thermalSolid", lb.set_TS_BC, lb.set_TS_BC_value)):
+                b = bc.get(key)
+                if not b:
+                    continue
+                bt = b.get("type")
+                if bt == "fixedValue":
+                    setter(i, BC.fixedValue)
+                    setv(i, b.get("value", 0.0))
+                elif bt == "zeroGradient":
+                    setter(i, BC.zeroGradient)
+                elif bt == "periodic":
+                    setter(i, BC.periodic)
+        if isChemical:
+            b = bc.get("chemical")
+            if b:
+                bt = b.get("type")
+                if bt == "fixedValue":
+                    for sp in lb.specieName:
+                        lb.set_specie_BC(sp, i, BC.fixedValue)
+                        lb.set_specie_BC_value(sp, i, b.get(sp, 0))
+                elif bt == "zeroGradient":
+                    lb.set_species_BC(i, BC.zeroGradient)
+                elif bt == "periodic":
+                    lb.set_species_BC(i, BC.periodic)
+
+    geo_dict = build_geometry(lb, config.get("geometry"), logger)
+    ic = config.get("initialCondition") or {}
+    fic = ic.get("flow")
+    if fic:
+        v = fic.get("velocity", [0, 0, 0])
+        lb.init_field(lb.rho, fic.get("rho", 1.0))
+        lb.init_field3(lb.v, float(v[0]), float(v[1]), float(v[2]) if len(v) > 2 else 0.0)
+    if isThermal and ic.get("thermal"):
+        tic = ic["thermal"]
+        T = tic.get("T", 0)
+        lb.init_field(lb.TF.S, lb.TF.get_normalized_value(tic.get("TF", T)))
+        lb.init_field(lb.TS.S, lb.TS.get_normalized_value(tic.get("TS", T)))
+    if isChemical and ic.get("chemical"):
+        for sp, c in ic["chemical"].items():
+            zone, value = c.get("zone"), c.get("value", 0)
+            if zone == "ALL":
+                lb.init_specie(sp, float(value))
+            else:
+                lb.init_specie(sp, value * _zone_sum(geo_dict, zone, 0))
+    for solid in (ic.get("solid") or {}).values():
+        st = solid.get("type")
+        occ = _zone_sum(geo_dict, solid.get("zone", "ALL"), 0) if geo_dict else None
+        if occ is None:
+            continue
+        s = (occ > 0) * 1.0
+        if st == "poro" and isPoro:
+            border = _zone_sum(geo_dict, solid.get("zone", "ALL"), 1)
+            lb.add_solid((1 - solid["porosity"]) * s)
+            lb.add_rho_solid(solid.get("rho", 1) * s)
+            if isThermal:
+                lb.set_heat_exchange_surface(solid.get("exchangeSurface", 1) * s)
+                lb.set_heat_exchange_coef(solid.get("exchangeCoef", 1) * s)
+                if isRadiation:
+                    lb.init_field(lb.TS.radiation_surface, border)
+                    lb.init_field(lb.TS.emissivity, border * solid.get("emisssivity", solid.get("emissivity", 1.0)))
+            pm = solid.get("porousModel")
+            if pm == "darcy":
+                lb.set_poro_Darcy(s, solid.get("darcy"))
+            elif pm == "darcyForchheimer":
+                lb.set_poro_Darcy_Forchheimer(s, solid.get("darcy"), solid.get("forchheimer"))
+        elif st == "concrete":
+            lb.add_solid(s)
+            lb.add_rho_solid(solid.get("rho", 1) * s)
+        elif st == "substract":
+            lb.add_solid(-s)
+    return lb
+
+
+def application(config, logger=None, lib=None, precision="fp32", collision="bgk"):
+    """Run a configuration to timeControl.endTime with logging, VTK export and
+    snapshots (app.py:562-677).  Returns the solver."""
+    logger = _log(logger)
+    t_start = time.time()
+    lb = configure(config, logger, lib=lib, precision=precision, collision=collision)
+    tc = config["timeControl"]
+    lb.tLattice = int(tc.get("startTime", 0) / lb.dt)
+    end = int(round(tc.get("endTime", 0) / lb.dt))
+    oc = config.get("outputControl") or {}
+    log_every = max(1, int(round((oc.get("log") or {}).get("interval", lb.dt) / lb.dt)))
+    vtk_c = oc.get("vtk") or {}
+    vtk_every = max(1, int(round(vtk_c.get("interval", 100) / lb.dt))) if vtk_c else None
+    lb.exportPath = vtk_c.get("path") or "output"
+    if vtk_c.get("clear"):
+        import shutil
+        shutil.rmtree(lb.exportPath, ignore_errors=True)
+    if vtk_c:
+        os.makedirs(lb.exportPath, exist_ok=True)
+    snap = oc.get("snapshot") or {}
+    snap_every = max(1, int(round(snap.get("interval", 100) / lb.dt))) if snap else None
+    if snap.get("path"):
+        lb.snapshotPath = snap["path"]
+
+    logger.info("Simulation initializing...")
+    lb.init_simulation()
+    logger.info(lb.description())
+    cells = lb.nx * lb.ny * lb.nz
+    t_prev, n_prev = time.time(), lb.tLattice
+    while lb.tLattice <= end:
+        t = lb.tLattice
+        if t % log_every == 0:
+            now = time.time()
+            dt_wall = now - t_prev
+            mlups = cells * (t - n_prev) / dt_wall / 1e6 if dt_wall > 0 and t > n_prev else 0.0
+            t_prev, n_prev = now, t
+            logger.info(f"Execution time:{dt_wall:.2f} s , Collapsed time:{now - t_start:.2f} s, MLUPS = {mlups:.2f}")
+            logger.info(lb.log_info())
+        if vtk_every and t % vtk_every == 0:
+            lb.export_VTK_pyvista()
+        if snap_every and t % snap_every == 0:
+            lb.export_snapshot(config)
+        if t == end:
+            break
+        # advance to the next event (or the end) in one batch
+        events = [end] + [((t // k) + 1) * k for k in (log_every, vtk_every, snap_every) if k]
+        lb.run(min(events) - t)
+    logger.info("LBM finished.")
+    return lb

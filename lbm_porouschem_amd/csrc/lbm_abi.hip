@@ -129,6 +129,21 @@ int lbm_sync(lbm_handle* h) {
 
 long long lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
 
+int lbm_state_size(const lbm_handle* h, size_t* bytes) {
+    if (!h || !bytes) return LBM_ERR_INVALID;
+    return h->impl->state_size(bytes);
+}
+
+int lbm_get_state(lbm_handle* h, void* host, size_t bytes) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    return h->impl->get_state(host, bytes);
+}
+
+int lbm_set_state(lbm_handle* h, const void* host, size_t bytes) {
+    if (!h || !host) return LBM_ERR_INVALID;
+    return h->impl->set_state(host, bytes);
+}
+
 int lbm_profile(lbm_handle* h, int enable) {
     if (!h) return LBM_ERR_INVALID;
     return h->impl->profile(enable);

@@ -231,6 +231,9 @@ struct Impl {
     virtual int step(int n) = 0;
     virtual int reduce(int kind, int comp, double* out) = 0;
     virtual int sync() = 0;
+    virtual int state_size(size_t* bytes) = 0;
+    virtual int get_state(void* host, size_t bytes) = 0;
+    virtual int set_state(const void* host, size_t bytes) = 0;
     virtual int profile(int enable) = 0;
     virtual int profile_read(int kernel, double* ms, long long* n) = 0;
 };
@@ -892,6 +895,88 @@ struct ImplT final : Impl {
 
     int sync() override {
         HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+
+    // ---- snapshot: every device array the step reads and writes, plus the step counter
+    struct Region {
+        void* p;
+        size_t bytes;
+    };
+    std::vector<Region> state_regions() {
+        std::vector<Region> r;
+        auto add = [&](const void* p, size_t b) {
+            if (p && b) r.push_back({const_cast<void*>(p), b});
+        };
+        const size_t NR = (size_t)N * sizeof(Real);
+        add(B.pops, (size_t)P.nslots * N * sizeof(Store));
+        add(B.fixS, nfix > 0 ? (size_t)nfix * NR : 0);
+        add(B.lag, (h_->d.thermal && h_->d.chemical) ? NR : 0);
+        add(B.lagTF, P.lag_tf ? NR : 0);
+        add(B.lagS, P.lag_sp ? (size_t)nf * NR : 0);
+        add(B.emit, has_emit ? (size_t)QS * NR : 0);
+        add(B.rho, NR);
+        add(B.u, 3 * NR);
+        if (h_->d.thermal) {
+            add(B.TF, NR);
+            add(B.TS, NR);
+            add(B.dTF, NR);
+            add(B.dTS, NR);
+            add(B.rhos, NR);
+        }
+        if (h_->d.n_species > 0) {
+            add(B.S, (size_t)h_->d.n_species * NR);
+            add(B.dS, (size_t)h_->d.n_species * NR);
+        }
+        long long nprof = 0;
+        for (int f = 0; f < 2 * D; ++f) nprof += prof_n[f];
+        add(B.vprof, 3 * (size_t)nprof * sizeof(Real));
+        add(B.rprof, (size_t)nprof * sizeof(Real));
+        return r;
+    }
+    static constexpr unsigned long long kStateMagic = 0x4c424d5354415445ull;  // "LBMSTATE"
+    int state_size(size_t* bytes) override {
+        if (!h_->initialised) return set_err(h_, LBM_ERR_STATE, "lbm_state_size before lbm_init");
+        size_t t = 4 * sizeof(unsigned long long);
+        for (auto& r : state_regions()) t += r.bytes;
+        *bytes = t;
+        return LBM_OK;
+    }
+    int get_state(void* host, size_t bytes) override {
+        size_t need;
+        int rc = state_size(&need);
+        if (rc) return rc;
+        if (bytes != need) return set_err(h_, LBM_ERR_INVALID, "state blob is %zu bytes, expected %zu", bytes, need);
+        unsigned long long* hd = (unsigned long long*)host;
+        hd[0] = kStateMagic;
+        hd[1] = (unsigned long long)need;
+        hd[2] = (unsigned long long)h_->steps;
+        hd[3] = (unsigned long long)(sizeof(Store) * 16 + sizeof(Real));
+        char* dst = (char*)host + 4 * sizeof(unsigned long long);
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (auto& r : state_regions()) {
+            HIP_TRY(hipMemcpyAsync(dst, r.p, r.bytes, hipMemcpyDeviceToHost, stream));
+            dst += r.bytes;
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        return LBM_OK;
+    }
+    int set_state(const void* host, size_t bytes) override {
+        size_t need;
+        int rc = state_size(&need);
+        if (rc) return rc;
+        const unsigned long long* hd = (const unsigned long long*)host;
+        if (bytes != need || hd[0] != kStateMagic || hd[1] != need ||
+            hd[3] != (unsigned long long)(sizeof(Store) * 16 + sizeof(Real)))
+            return set_err(h_, LBM_ERR_INVALID, "state blob does not match this handle");
+        const char* src = (const char*)host + 4 * sizeof(unsigned long long);
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (auto& r : state_regions()) {
+            HIP_TRY(hipMemcpyAsync(r.p, src, r.bytes, hipMemcpyHostToDevice, stream));
+            src += r.bytes;
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        h_->steps = (long long)hd[2];
         return LBM_OK;
     }
 

@@ -1,0 +1,172 @@
+"""Geometry inputs of the solver: voxelised shapes, STL voxelisation, seeded packings
+and the surface weights of the radiation / exchange surface.
+
+The reference builds unit shapes with CadQuery and samples them with pyvista
+(GEO/STL.py:31-95 unit sphere / cylinder / box / cone, LBM3D/_io.py:490-505
+load_stl); neither library is available here, so shapes are voxelised directly:
+a cell is inside when its centre, (i + 1/2) dx along each axis, lies inside the
+shape.  STL files (ASCII or binary) are voxelised by ray parity along z.
+
+`surface_weight` restates the neighbour-count table of GEO/STL.py:216-259: a solid
+cell with m of its 2*dim face neighbours solid gets the weight
+  2D  m = 4,3,2,1,0 -> 0, 1, 1.414, 2, 4
+  3D  m = 6,5,4,3,2,1,0 -> 0, 1, 1.414, 2, 3, 4, 6
+(cells on the domain border count as non-surface, as in the reference).
+"""
+import struct
+
+import numpy as np
+
+_W2 = {4: 0.0, 3: 1.0, 2: 1.414, 1: 2.0, 0: 4.0}
+_W3 = {6: 0.0, 5: 1.0, 4: 1.414, 3: 2.0, 2: 3.0, 1: 4.0, 0: 6.0}
+
+
+def cell_centres(shape, dx):
+    """Physical coordinates of the cell centres, each array of `shape`."""
+    axes = [(np.arange(n) + 0.5) * dx for n in shape]
+    return np.meshgrid(*axes, indexing="ij")
+
+
+def _rotation_to(axis):
+    """Rotation matrix taking the unit z axis to `axis`."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    z = np.array([0.0, 0.0, 1.0])
+    v = np.cross(z, a)
+    c = float(np.dot(z, a))
+    if np.linalg.norm(v) < 1e-12:
+        return np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])
+    vx = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+    return np.eye(3) + vx + vx @ vx * (1.0 / (1.0 + c))
+
+
+def _local(shape, dx, center, axis):
+    X, Y, Z = cell_centres(shape, dx) if len(shape) == 3 else (*cell_centres(shape, dx), None)
+    p = np.stack([X - center[0], Y - center[1], (Z - center[2]) if Z is not None else np.zeros_like(X)], axis=-1)
+    R = _rotation_to(axis)
+    return p @ R  # coordinates in the frame whose z axis is `axis`
+
+
+def sphere(shape, dx, center, radius):
+    """Sphere (3D) or disk (2D: shape has two axes) of the given radius."""
+    c = cell_centres(shape, dx)
+    r2 = sum((ci - center[i]) ** 2 for i, ci in enumerate(c))
+    return (r2 <= radius * radius).astype(np.float64)
+
+
+def cylinder(shape, dx, center, radius, height, axis=(0, 0, 1)):
+    q = _local(shape, dx, center, axis)
+    return ((q[..., 0] ** 2 + q[..., 1] ** 2 <= radius * radius) & (np.abs(q[..., 2]) <= height / 2)).astype(np.float64)
+
+
+def cone(shape, dx, center, radius, height, axis=(0, 0, 1)):
+    """Cone with its base (radius) at -height/2 and apex at +height/2 along `axis`."""
+    q = _local(shape, dx, center, axis)
+    h = q[..., 2] + height / 2
+    rr = radius * (1 - h / height)
+    return ((h >= 0) & (h <= height) & (q[..., 0] ** 2 + q[..., 1] ** 2 <= rr * rr)).astype(np.float64)
+
+
+def box(shape, dx, center, size):
+    c = cell_centres(shape, dx)
+    inside = np.ones(shape, dtype=bool)
+    for i, ci in enumerate(c):
+        inside &= np.abs(ci - center[i]) <= size[i] / 2
+    return inside.astype(np.float64)
+
+
+def read_stl(path):
+    """Triangles (T, 3, 3) of an ASCII or binary STL file."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:5] == b"solid" and b"facet" in data[:512]:
+        verts = []
+        for line in data.decode("ascii", errors="replace").splitlines():
+            t = line.split()
+            if t and t[0] == "vertex":
+                verts.append([float(t[1]), float(t[2]), float(t[3])])
+        return np.asarray(verts, dtype=np.float64).reshape(-1, 3, 3)
+    n = struct.unpack("<I", data[80:84])[0]
+    rec = np.frombuffer(data, dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]), count=n,
+                        offset=84)
+    return rec["v"].astype(np.float64)
+
+
+def voxelize_triangles(tri, shape, dx):
+    """Cells whose centre is inside the closed triangle mesh (ray parity along +z)."""
+    X, Y = np.meshgrid((np.arange(shape[0]) + 0.5) * dx, (np.arange(shape[1]) + 0.5) * dx, indexing="ij")
+    zc = (np.arange(shape[2]) + 0.5) * dx
+    crossings = [[[] for _ in range(shape[1])] for _ in range(shape[0])]
+    for t in tri:
+        (x0, y0, z0), (x1, y1, z1), (x2, y2, z2) = t
+        xmin, xmax = min(x0, x1, x2), max(x0, x1, x2)
+        ymin, ymax = min(y0, y1, y2), max(y0, y1, y2)
+        i0, i1 = max(int(np.floor(xmin / dx - 0.5)), 0), min(int(np.ceil(xmax / dx - 0.5)), shape[0] - 1)
+        j0, j1 = max(int(np.floor(ymin / dx - 0.5)), 0), min(int(np.ceil(ymax / dx - 0.5)), shape[1] - 1)
+        det = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2)
+        if abs(det) < 1e-300 or i0 > i1 or j0 > j1:
+            continue
+        px, py = X[i0:i1 + 1, j0:j1 + 1], Y[i0:i1 + 1, j0:j1 + 1]
+        a = ((y1 - y2) * (px - x2) + (x2 - x1) * (py - y2)) / det
+        b = ((y2 - y0) * (px - x2) + (x0 - x2) * (py - y2)) / det
+        c = 1 - a - b
+        hit = (a >= 0) & (b >= 0) & (c >= 0)
+        zs = a * z0 + b * z1 + c * z2
+        for ii, jj in zip(*np.nonzero(hit)):
+            crossings[i0 + ii][j0 + jj].append(zs[ii, jj])
+    out = np.zeros(shape, dtype=np.float64)
+    for i in range(shape[0]):
+        for j in range(shape[1]):
+            zs = np.unique(np.round(np.asarray(crossings[i][j]), 12))
+            if zs.size >= 2:
+                count = np.searchsorted(zs, zc, side="right")  # crossings below each centre
+                out[i, j, :] = (count % 2 == 1)
+    return out
+
+
+def stl(shape, dx, path, scale=(1, 1, 1), translate=(0, 0, 0)):
+    tri = read_stl(path) * np.asarray(scale, dtype=np.float64) + np.asarray(translate, dtype=np.float64)
+    return voxelize_triangles(tri, shape, dx)
+
+
+def packing(shape, dx, rmin, rmax, fraction, seed):
+    """Seeded overlapping spheres (3D) / disks (2D) with radii uniform in [rmin, rmax]
+    (physical units), placed until the covered fraction reaches `fraction`."""
+    rng = np.random.default_rng(seed)
+    mask = np.zeros(shape, dtype=bool)
+    c = cell_centres(shape, dx)
+    ext = [n * dx for n in shape]
+    for _ in range(100000):
+        if mask.mean() >= fraction:
+            break
+        r = rng.uniform(rmin, rmax)
+        ctr = [rng.uniform(0, e) for e in ext]
+        r2 = sum((ci - ctr[i]) ** 2 for i, ci in enumerate(c))
+        mask |= r2 <= r * r
+    return mask.astype(np.float64)
+
+
+def surface_weight(voxels):
+    """Radiation / exchange surface weight of each solid cell from its solid face
+    neighbours (table in the module docstring); zero outside solids and on the
+    domain border."""
+    v = np.asarray(voxels) > 0
+    dim = 2 if v.ndim == 2 or (v.ndim == 3 and v.shape[2] == 1) else 3
+    vv = v[..., 0] if (v.ndim == 3 and dim == 2) else v
+    pad = np.pad(vv, 1)
+    count = np.zeros(vv.shape, dtype=np.int64)
+    for ax in range(dim):
+        for s in (-1, 1):
+            count += np.roll(pad, s, axis=ax)[tuple(slice(1, -1) for _ in range(dim))]
+    table = _W3 if dim == 3 else _W2
+    w = np.vectorize(lambda m: table[int(m)])(count).astype(np.float64)
+    w[~vv] = 0.0
+    border = np.zeros(vv.shape, dtype=bool)
+    for ax in range(dim):
+        idx = [slice(None)] * dim
+        idx[ax] = 0
+        border[tuple(idx)] = True
+        idx[ax] = -1
+        border[tuple(idx)] = True
+    w[border] = 0.0
+    return w.reshape(v.shape)

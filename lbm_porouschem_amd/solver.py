@@ -14,6 +14,8 @@ fields that feed the populations can only be changed before initialisation.
 `step()` is lazy when no UpdateBCfunc callback is registered: steps are queued
 and launched as one batch the next time anything is observed.
 """
+import os
+
 import numpy as np
 
 from . import _abi
@@ -126,6 +128,9 @@ class SmallField:
         return self._a[i]
 
     def __setitem__(self, i, v):
+        if self._a.ndim == 2 and np.ndim(v) == 1 and len(v) < self._a.shape[1]:
+            # 2D vectors (the 2D solver's v_BC holds (vx, vy)); the z component is zero
+            v = list(v) + [0.0] * (self._a.shape[1] - len(v))
         self._a[i] = v
 
     def to_numpy(self):
@@ -788,8 +793,26 @@ class LBMSolverBase:
     def set_specie_BCs_flux(self, specie, fs):
         self._sp(specie).set_s_BCs_flux(fs)
 
-    def load_stl(self, *args, **kwargs):
-        raise NotImplementedError("STL voxelisation is not part of this package; pass a solid array to add_solid()")
+    def load_stl(self, stl_path, scale=(1.0, 1.0, 1.0), translate=(0, 0, 0), rotate=(0, 0, 0), **kwargs):
+        """Voxelise an STL file on the lattice (LBM3D/_io.py:490-505).  Returns the
+        (nx, ny, nz) occupancy array and its surface weights (geometry.surface_weight).
+        `rotate` holds x-y-z Euler angles in degrees applied about the mesh origin
+        before scaling and translation."""
+        from . import geometry
+        tri = geometry.read_stl(stl_path)
+        if any(float(a) != 0.0 for a in rotate):
+            from scipy.spatial.transform import Rotation
+            R = Rotation.from_euler("xyz", [float(a) for a in rotate], degrees=True).as_matrix()
+            tri = tri @ R.T
+        tri = tri * np.asarray(scale, dtype=np.float64) + np.asarray(translate, dtype=np.float64)
+        shape = (self.nx, self.ny, self.nz)
+        if self._dim == 2:
+            # extrude the 2D domain: voxelise the mid-plane of a one-cell-thick slab
+            vox = geometry.voxelize_triangles(tri, (self.nx, self.ny, 1), self.dx)
+        else:
+            vox = geometry.voxelize_triangles(tri, shape, self.dx)
+        vox = vox.reshape(shape)
+        return vox, geometry.surface_weight(vox)
 
     # ------------------------------------------------------------------
     # device lifecycle
@@ -1008,9 +1031,75 @@ class LBMSolverBase:
             p += f"Min temperature: {self.get_min_T():.7f} K, Max temperature : {self.get_max_T():.7f}"
         return p
 
-    def export_snapshot(self, config):
-        """The reference's export_snapshot is a no-op (LBM3D/_io.py:593-595)."""
-        pass
+    def get_data_pyvista(self):
+        """Point data of the VTK export, named and flattened (Fortran order) like the
+        reference's get_data_pyvista (LBM3D/_io.py:710-751)."""
+        data = {"solid": self.solid.to_numpy().ravel(order="F"),
+                "rho": self.rho.to_numpy().ravel(order="F"),
+                "velocity": self.v.to_numpy().reshape(-1, 3, order="F")}
+        if self.TEMPERATURE:
+            data["rho_solid"] = self.rhos.to_numpy().ravel(order="F")
+            data["Tf"] = self.TF.get_physical_value(self.TF.S.to_numpy().ravel(order="F"))
+            data["Ts"] = self.TS.get_physical_value(self.TS.S.to_numpy().ravel(order="F"))
+            data["dTf"] = self.TF.dS.to_numpy().ravel(order="F")
+            data["dTs"] = self.TS.dS.to_numpy().ravel(order="F")
+            data["solid_fluid_exchange_surface"] = self.TS.exchangeSurface.to_numpy().ravel(order="F")
+            data["solid_fluid_exchange_coef"] = self.TS.exchangeCoef.to_numpy().ravel(order="F")
+            if self.RADIATION:
+                data["radiation_surface"] = self.TS.radiation_surface.to_numpy().ravel(order="F")
+                data["emissivity"] = self.TS.emissivity.to_numpy().ravel(order="F")
+        if self.CHEMISTRY:
+            for sp in self.species:
+                if sp.FIX and not sp.name.endswith("(S)"):
+                    data[sp.name + "(S)"] = sp.S.to_numpy().ravel(order="F")
+                else:
+                    data[sp.name] = sp.S.to_numpy().ravel(order="F")
+                    data["d " + sp.name] = sp.dS.to_numpy().ravel(order="F")
+        return data
+
+    def export_VTK_pyvista(self):
+        """Write exportPath/name_t.vts and update exportPath/name.pvd (the reference's
+        export_VTK_pyvista, LBM3D/_io.py:688-702, without pyvista)."""
+        from . import vtk
+        os.makedirs(self.exportPath, exist_ok=True)
+        if getattr(self, "PVD", None) is None or self.PVD.exportPath != self.exportPath:
+            self.PVD = vtk.PVD(self.exportPath, self.name)
+        fn = f"{self.name}_{self.tLattice}.vts"
+        vtk.write_vts(os.path.join(self.exportPath, fn), (self.nx, self.ny, self.nz), self.dx, self.get_data_pyvista())
+        self.PVD.addVTK(self.tLattice * self.dt, fn)
+        self.PVD.writePVD()
+
+    export_VTK = export_VTK_pyvista
+
+    def export_snapshot(self, config=None, path=None):
+        """Save the complete simulation state (device populations, FIX densities, lagged
+        fields, step counter) plus `config` next to snapshotPath.  The reference's
+        export_snapshot is a stub (LBM3D/_io.py:593-595); load_snapshot resumes a run
+        bit-for-bit on the same build and configuration."""
+        import json
+        path = path or self.snapshotPath
+        if path.endswith(".yaml") or path.endswith(".json"):
+            path = path.rsplit(".", 1)[0]
+        self._flush()
+        blob = self._lib.get_state(self._handle)
+        np.savez(path + ".npz", state=np.frombuffer(blob, dtype=np.uint8), tLattice=np.int64(self.tLattice))
+        with open(path + ".json", "w") as f:
+            json.dump({"tLattice": int(self.tLattice), "shape": [self.nx, self.ny, self.nz], "name": self.name,
+                       "precision": self.precision, "config": config}, f, default=str)
+        return path + ".npz"
+
+    def load_snapshot(self, path):
+        """Restore the state written by export_snapshot into an initialised solver of the
+        same configuration."""
+        path = path.rsplit(".", 1)[0] if path.endswith((".npz", ".json", ".yaml")) else path
+        if self._handle is None:
+            self.init_simulation()
+        self._flush()
+        with np.load(path + ".npz") as z:
+            state = z["state"].tobytes()
+            t = int(z["tLattice"])
+        self._lib.set_state(self._handle, state)
+        self.tLattice = t
 
 
 class LBM3DSolver(LBMSolverBase):

@@ -1265,6 +1265,89 @@ int lbm_reduce(lbm_handle* h, int kind, int comp, double* out) {
 
 int lbm_sync(lbm_handle* h) { return h ? LBM_OK : LBM_ERR_INVALID; }
 long long lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+/* snapshot: every array the step reads and writes, in a fixed order */
+typedef struct { void* p; size_t bytes; } region;
+static int state_regions(const lbm_handle* h, region* r) {
+    int n = 0;
+    const size_t N = h->N, Q = (size_t)h->Q, Qs = (size_t)h->Qs, R = sizeof(real);
+#define ADD(ptr, b)                          \
+    do {                                     \
+        if ((ptr) && (b) != 0) {             \
+            r[n].p = (void*)(ptr);           \
+            r[n].bytes = (b);                \
+            ++n;                             \
+        }                                    \
+    } while (0)
+    ADD(h->f, Q * N * R);
+    ADD(h->F, Q * N * R);
+    ADD(h->rho, N * R);
+    ADD(h->v, 3 * N * R);
+    ADD(h->rhos, N * R);
+    const scalar_lattice* ls[2] = {&h->TF, &h->TS};
+    for (int t = 0; t < 2; ++t) {
+        ADD(ls[t]->S, N * R);
+        ADD(ls[t]->dS, N * R);
+        ADD(ls[t]->g, Qs * N * R);
+        ADD(ls[t]->G, Qs * N * R);
+    }
+    for (int j = 0; j < h->d.n_species; ++j) {
+        ADD(h->sp[j].S, N * R);
+        ADD(h->sp[j].dS, N * R);
+        ADD(h->sp[j].g, Qs * N * R);
+        ADD(h->sp[j].G, Qs * N * R);
+    }
+    for (int f = 0; f < 6; ++f) {
+        ADD(h->vprof[f], 3 * h->face_n[f] * R);
+        ADD(h->rprof[f], h->face_n[f] * R);
+    }
+#undef ADD
+    return n;
+}
+int lbm_state_size(const lbm_handle* h, size_t* bytes) {
+    if (!h || !bytes) return LBM_ERR_INVALID;
+    if (!h->initialised) return LBM_ERR_STATE;
+    region r[128];
+    int n = state_regions(h, r);
+    size_t t = 2 * sizeof(long long);
+    for (int i = 0; i < n; ++i) t += r[i].bytes;
+    *bytes = t;
+    return LBM_OK;
+}
+int lbm_get_state(lbm_handle* h, void* host, size_t bytes) {
+    size_t need;
+    int rc = lbm_state_size(h, &need);
+    if (rc) return rc;
+    if (!host || bytes != need) return set_err(h, LBM_ERR_INVALID, "state blob is %zu bytes, expected %zu", bytes, need);
+    long long* hd = (long long*)host;
+    hd[0] = (long long)need;
+    hd[1] = h->steps;
+    char* dst = (char*)host + 2 * sizeof(long long);
+    region r[128];
+    int n = state_regions(h, r);
+    for (int i = 0; i < n; ++i) {
+        memcpy(dst, r[i].p, r[i].bytes);
+        dst += r[i].bytes;
+    }
+    return LBM_OK;
+}
+int lbm_set_state(lbm_handle* h, const void* host, size_t bytes) {
+    size_t need;
+    int rc = lbm_state_size(h, &need);
+    if (rc) return rc;
+    const long long* hd = (const long long*)host;
+    if (!host || bytes != need || hd[0] != (long long)need)
+        return set_err(h, LBM_ERR_INVALID, "state blob does not match this handle");
+    const char* src = (const char*)host + 2 * sizeof(long long);
+    region r[128];
+    int n = state_regions(h, r);
+    for (int i = 0; i < n; ++i) {
+        memcpy(r[i].p, src, r[i].bytes);
+        src += r[i].bytes;
+    }
+    h->steps = hd[1];
+    return LBM_OK;
+}
 int lbm_profile(lbm_handle* h, int enable) { (void)enable; return h ? LBM_OK : LBM_ERR_INVALID; }
 int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* launches) {
     (void)kernel;

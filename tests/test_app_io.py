@@ -1,0 +1,207 @@
+"""CPU tests of the callers around the step: geometry voxelisation and surface
+weights, the VTK writer, the YAML application loop, and snapshot / resume.  The
+application is driven through the CPU checker library so no GPU is needed (the
+product library runs the same code path on the GPU in test_gpu_app.py)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from lbm_porouschem_amd import app, geometry, vtk
+
+
+# ---------------------------------------------------------------------------
+# geometry
+def test_sphere_and_disk_volume():
+    v = geometry.sphere((40, 40, 40), 0.05, (1.0, 1.0, 1.0), 0.6)
+    assert abs(v.sum() * 0.05 ** 3 - 4 / 3 * np.pi * 0.6 ** 3) <= 0.03 * 4 / 3 * np.pi * 0.6 ** 3
+    d = geometry.sphere((60, 60), 0.05, (1.5, 1.5), 0.8)
+    assert abs(d.sum() * 0.05 ** 2 - np.pi * 0.8 ** 2) <= 0.02 * np.pi * 0.8 ** 2
+
+
+def test_cylinder_axis_rotates_the_shape():
+    z = geometry.cylinder((20, 20, 20), 0.1, (1.0, 1.0, 1.0), 0.3, 1.2, axis=(0, 0, 1))
+    x = geometry.cylinder((20, 20, 20), 0.1, (1.0, 1.0, 1.0), 0.3, 1.2, axis=(1, 0, 0))
+    assert z.sum() == x.sum() > 0
+    assert np.array_equal(np.transpose(z, (2, 1, 0)), x)
+
+
+def test_cone_is_narrower_at_the_apex():
+    c = geometry.cone((20, 20, 20), 0.1, (1.0, 1.0, 1.0), 0.8, 1.6, axis=(0, 0, 1))
+    per_z = c.sum(axis=(0, 1))
+    nz = np.nonzero(per_z)[0]
+    assert per_z[nz[0]] > per_z[nz[-1]]
+
+
+def _cube_stl(path, lo, hi):
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]])
+    faces = [(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4), (2, 3, 7), (2, 7, 6),
+             (1, 2, 6), (1, 6, 5), (0, 4, 7), (0, 7, 3)]
+    with open(path, "w") as f:
+        f.write("solid cube\n")
+        for a, b, c in faces:
+            f.write("facet normal 0 0 0\nouter loop\n")
+            for i in (a, b, c):
+                f.write(f"vertex {v[i][0]} {v[i][1]} {v[i][2]}\n")
+            f.write("endloop\nendfacet\n")
+        f.write("endsolid cube\n")
+
+
+def test_stl_voxelisation_of_a_cube(tmp_path):
+    p = str(tmp_path / "cube.stl")
+    _cube_stl(p, (0.2, 0.3, 0.1), (0.6, 0.7, 0.5))
+    vox = geometry.stl((10, 10, 10), 0.1, p)
+    ref = geometry.box((10, 10, 10), 0.1, (0.4, 0.5, 0.3), (0.4, 0.4, 0.4))
+    assert np.array_equal(vox, ref) and vox.sum() == 64
+
+
+def test_surface_weight_table():
+    # reference table GEO/STL.py:216-259: isolated solid cell 6 (3D) / 4 (2D); full block interior 0
+    v = np.zeros((7, 7, 7))
+    v[3, 3, 3] = 1
+    assert geometry.surface_weight(v)[3, 3, 3] == 6.0
+    v[2:5, 2:5, 2:5] = 1
+    w = geometry.surface_weight(v)
+    assert w[3, 3, 3] == 0.0 and w[2, 3, 3] == 1.0 and w[2, 2, 3] == 1.414 and w[2, 2, 2] == 2.0
+    d = np.zeros((6, 6))
+    d[2, 2] = 1
+    assert geometry.surface_weight(d)[2, 2] == 4.0
+    d[2, 3] = 1
+    assert geometry.surface_weight(d)[2, 2] == 2.0
+
+
+def test_packing_reaches_its_fraction_and_is_seeded():
+    a = geometry.packing((40, 40, 40), 1.0, 3, 6, 0.3, seed=4)
+    b = geometry.packing((40, 40, 40), 1.0, 3, 6, 0.3, seed=4)
+    assert np.array_equal(a, b) and 0.3 <= a.mean() < 0.5
+
+
+# ---------------------------------------------------------------------------
+# VTK writer
+def test_vts_roundtrip_and_pvd(tmp_path):
+    shape = (3, 4, 5)
+    rng = np.random.default_rng(0)
+    data = {"rho": rng.random(60), "velocity": rng.random((60, 3)), "Tf": rng.random(60).astype(np.float32)}
+    path = str(tmp_path / "a_0.vts")
+    vtk.write_vts(path, shape, 0.1, data)
+    back = vtk.read_vts(path)
+    for k in data:
+        assert np.array_equal(back[k], data[k])
+    pvd = vtk.PVD(str(tmp_path), "a")
+    pvd.addVTK(0.0, "a_0.vts")
+    pvd.addVTK(0.5, "a_5.vts")
+    pvd.writePVD()
+    text = open(tmp_path / "a.pvd").read()
+    assert 'timestep="0.5"' in text and 'file="a_5.vts"' in text
+
+
+# ---------------------------------------------------------------------------
+# application
+CONFIG = """
+basic: {name: tiny, dimension: 2, arch: cpu}
+module: {flow: true, porous_media: true, temperature: true, chemistry: true, radiation: false}
+spaceControl: {geometry: [0.003, 0.0016], DX: 1.0e-4}
+timeControl: {DT: 1.0e-4, startTime: 0, endTime: 0.0012}
+flowProperties: {viscosity: {type: constant, value: 1.0e-5}}
+thermalPropertiesFluid:
+  normalize: {Trange: [300, 800]}
+  thermalDiff: {type: Prandtl, Pr: 0.71}
+  capacity: {type: constant, value: 1000}
+thermalPropertiesSolid:
+  normalize: {Trange: [300, 800]}
+  thermalDiff: {type: constant, value: 2.0e-6}
+  capacity: {type: constant, value: 1500}
+chemicalProperties:
+  type: input
+  species:
+    N2: {composition: {N: 2}, state: fluid, diffusitivity: {type: constant, value: 2.0e-5}}
+    gas: {composition: {C: 1, O: 1}, state: fluid, diffusitivity: {type: constant, value: 2.0e-5}}
+    biomass: {state: solid}
+  reactions:
+    - {equation: biomass => gas, rate-constant: {A: 2.0, Ea: 1.0e4, b: 0}}
+boundaryCondition:
+  left: {flow: {type: inlet, velocity: [0.02, 0]}, thermalFluid: {type: fixedValue, value: 800},
+         chemical: {type: fixedValue, N2: 1.0}}
+  right: {flow: {type: outlet, rho: 1.0}, thermalFluid: {type: zeroGradient}, chemical: {type: zeroGradient}}
+  down: {flow: {type: wall}, thermalFluid: {type: zeroGradient}}
+  up: {flow: {type: wall}, thermalFluid: {type: zeroGradient}}
+geometry:
+  pellet: {type: sphere, center: [0.0015, 0.0008, 0], radius: 0.0004}
+initialCondition:
+  flow: {rho: 1.0, velocity: [0, 0, 0]}
+  thermal: {T: 300}
+  chemical:
+    N2: {zone: ALL, value: 1.0}
+    biomass: {zone: pellet, value: 1.0}
+  solid:
+    pellet: {type: poro, porosity: 0.5, zone: pellet, rho: 1.0, exchangeSurface: 1.0, exchangeCoef: 100,
+             porousModel: darcy, darcy: 1.0e6}
+outputControl:
+  log: {interval: 0.0004}
+  vtk: {interval: 0.0006, path: OUT}
+  snapshot: {interval: 0.0006, path: SNAP}
+"""
+
+
+def _config(tmp_path):
+    c = app.load_config(CONFIG)
+    c["outputControl"]["vtk"]["path"] = str(tmp_path / "vtk")
+    c["outputControl"]["snapshot"]["path"] = str(tmp_path / "snap")
+    return c
+
+
+def test_application_runs_and_writes_outputs(tmp_path, oracle64):
+    lb = app.application(_config(tmp_path), lib=oracle64, precision="fp64")
+    assert lb.tLattice == 12
+    files = sorted(os.listdir(tmp_path / "vtk"))
+    assert files == ["tiny.pvd", "tiny_0.vts", "tiny_12.vts", "tiny_6.vts"]
+    d = vtk.read_vts(str(tmp_path / "vtk" / "tiny_12.vts"))
+    assert set(["solid", "rho", "velocity", "Tf", "Ts", "N2", "gas", "biomass(S)"]) <= set(d)
+    assert d["Tf"].max() > 500  # the heated inlet reached the field
+    assert abs(d["solid"].max() - 0.5) < 1e-12  # porosity 0.5 inside the pellet
+    meta = json.load(open(str(tmp_path / "snap") + ".json"))
+    assert meta["tLattice"] == 12 and meta["config"]["basic"]["name"] == "tiny"
+
+
+def test_snapshot_resume_is_exact(tmp_path, oracle64):
+    cfg = _config(tmp_path)
+    A = app.configure(cfg, lib=oracle64, precision="fp64")
+    A.init_simulation()
+    A.run(5)
+    A.export_snapshot(cfg, path=str(tmp_path / "s5"))
+    A.run(7)
+    B = app.configure(cfg, lib=oracle64, precision="fp64")
+    B.load_snapshot(str(tmp_path / "s5.npz"))
+    assert B.tLattice == 5
+    B.run(7)
+    for fa, fb in ((A.v, B.v), (A.rho, B.rho), (A.TF.S, B.TF.S), (A.TS.S, B.TS.S)):
+        assert np.array_equal(fa.to_numpy(), fb.to_numpy())
+    for sa, sb in zip(A.species, B.species):
+        assert np.array_equal(sa.S.to_numpy(), sb.S.to_numpy())
+
+
+def test_snapshot_of_another_configuration_is_refused(tmp_path, oracle64):
+    cfg = _config(tmp_path)
+    A = app.configure(cfg, lib=oracle64, precision="fp64")
+    A.init_simulation()
+    A.export_snapshot(cfg, path=str(tmp_path / "s0"))
+    cfg2 = _config(tmp_path)
+    cfg2["spaceControl"]["geometry"] = [0.0032, 0.0016]
+    B = app.configure(cfg2, lib=oracle64, precision="fp64")
+    with pytest.raises(Exception):
+        B.load_snapshot(str(tmp_path / "s0.npz"))
+
+
+def test_yaml12_floats():
+    c = app.load_config("a: 2e-5\nb: 1.0e6\nc: -3E+2\nd: 7\ne: text\nf: 0.5")
+    assert c == {"a": 2e-5, "b": 1e6, "c": -300.0, "d": 7, "e": "text", "f": 0.5}
+
+
+def test_cli_help():
+    from click.testing import CliRunner
+    from lbm_porouschem_amd import cli
+    r = CliRunner().invoke(cli.run, ["--help"])
+    assert r.exit_code == 0 and "--config" in r.output

@@ -1,0 +1,41 @@
+"""The YAML application and snapshot / resume on the HIP library (GPU)."""
+import numpy as np
+import pytest
+
+from cases import rel_l2
+from lbm_porouschem_amd import app, vtk
+from test_app_io import _config
+
+pytestmark = pytest.mark.gpu
+
+
+def test_application_matches_checker(tmp_path, hip, oracle64):
+    print("[gpu app] tiny 2D configuration fp64", flush=True)
+    cfg = _config(tmp_path)
+    A = app.application(cfg, lib=hip, precision="fp64")
+    cfg_b = _config(tmp_path / "b")
+    B = app.application(cfg_b, lib=oracle64, precision="fp64")
+    assert A.tLattice == B.tLattice == 12
+    for fa, fb in ((A.v, B.v), (A.rho, B.rho), (A.TF.S, B.TF.S), (A.TS.S, B.TS.S)):
+        assert rel_l2(fa.to_numpy(), fb.to_numpy()) <= 1e-10
+    da = vtk.read_vts(str(tmp_path / "vtk" / "tiny_12.vts"))
+    db = vtk.read_vts(str(tmp_path / "b" / "vtk" / "tiny_12.vts"))
+    assert rel_l2(da["Tf"], db["Tf"]) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_snapshot_resume_is_bit_exact(tmp_path, hip, precision):
+    print(f"[gpu snapshot] tiny 2D configuration {precision}", flush=True)
+    cfg = _config(tmp_path)
+    A = app.configure(cfg, lib=hip, precision=precision)
+    A.init_simulation()
+    A.run(5)
+    A.export_snapshot(cfg, path=str(tmp_path / "s5"))
+    A.run(8)
+    B = app.configure(cfg, lib=hip, precision=precision)
+    B.load_snapshot(str(tmp_path / "s5.npz"))
+    B.run(8)
+    for fa, fb in ((A.v, B.v), (A.rho, B.rho), (A.TF.S, B.TF.S), (A.TS.S, B.TS.S)):
+        assert np.array_equal(fa.to_numpy(), fb.to_numpy())
+    for sa, sb in zip(A.species, B.species):
+        assert np.array_equal(sa.S.to_numpy(), sb.S.to_numpy())
