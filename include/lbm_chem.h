@@ -1,0 +1,165 @@
+/*
+ * lbm_chem.h -- C ABI of the cell-local chemistry operator of the coupled step:
+ * detailed gas-phase kinetics plus biomass pyrolysis, integrated per cell over one
+ * lattice time step (the plain FP64 stiff path), and its learned replacement (an
+ * MLP evaluated with BF16 matrix-core kernels, with per-cell fallback to the plain
+ * integrator), plus the training kernels of that MLP.
+ *
+ * The reference has no counterpart for this operator: its reaction source is the
+ * explicit single-step Arrhenius update of Reaction.reaction / Reactions.update_dS
+ * (src/LBM/LBM3D/_chemical.py:204-270), which the step kernels of
+ * lbm_porouschem.h restate.  This operator is an additional source term that a
+ * lattice handle can carry (lbm_chem_attach); with it detached the step is the
+ * reference's step unchanged.
+ *
+ * Exported by liblbm_mi355x.so (HIP kernels) and, for the entry points marked
+ * [oracle], by the CPU checker liblbm_oracle.so with identical semantics.
+ *
+ * Cell state (the 16 operator inputs, in this order):
+ *   0      T      temperature [K]
+ *   1      p      pressure [Pa]
+ *   2..13  Y_k    gas mass fractions of the 12 species of the mechanism
+ *   14     rho_B  biomass density [kg/m^3]
+ *   15     rho_C  char density [kg/m^3]
+ * Operator outputs over one step dt (the 14 values the surrogate predicts):
+ *   0..11  d rho_k  gas partial-density change [kg/m^3]
+ *   12     d rho_B  biomass density change [kg/m^3]
+ *   13     dT       temperature change [K]
+ * The char change follows from mass conservation, d rho_C = -d rho_B - sum_k d rho_k.
+ *
+ * Array arguments of the device entry points are device pointers, structure of
+ * arrays: value v of cell i at [v * n + i].
+ */
+#ifndef LBM_CHEM_H
+#define LBM_CHEM_H
+
+#include <stddef.h>
+
+#include "lbm_porouschem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_CHEM_NSP 12     /* gas species of the operator */
+#define LBM_CHEM_MAXR 48    /* gas-phase reactions */
+#define LBM_CHEM_NPYRO 3    /* competing pyrolysis reactions */
+#define LBM_CHEM_NIN 16     /* operator inputs */
+#define LBM_CHEM_NOUT 14    /* operator outputs */
+#define LBM_MLP_HIDDEN 128  /* surrogate: 16 -> 128 -> 128 -> 14, ReLU */
+/* parameter vector of the MLP (FP32 master copy), in this order:
+ * W1[128][16], b1[128], W2[128][128], b2[128], W3[14][128], b3[14] */
+#define LBM_MLP_NPARAM (128 * 16 + 128 + 128 * 128 + 128 + 14 * 128 + 14)
+
+#define LBM_CHEM_RXN_ELEMENTARY 0
+#define LBM_CHEM_RXN_THREEBODY 1
+#define LBM_CHEM_RXN_LINDEMANN 2
+#define LBM_CHEM_RXN_TROE 3
+
+#define LBM_CHEM_OFF 0        /* lbm_chem_attach modes */
+#define LBM_CHEM_PLAIN 1      /* FP64 stiff integration in every cell */
+#define LBM_CHEM_SURROGATE 2  /* MLP; plain integration where a cell leaves the hull or fails the residual */
+
+typedef struct lbm_chem_mech {
+    int n_species;                        /* must be LBM_CHEM_NSP */
+    int n_reactions;                      /* <= LBM_CHEM_MAXR */
+    double molemass[LBM_CHEM_NSP];        /* kg/mol */
+    double elements[LBM_CHEM_NSP][4];     /* atoms of C, H, O, N per molecule */
+    double nasa_T[LBM_CHEM_NSP][3];       /* NASA7 ranges: low, mid, high [K] */
+    double nasa[LBM_CHEM_NSP][2][7];      /* NASA7 coefficients below / above the mid temperature */
+    /* gas-phase reactions, all reversible (reverse rate from the equilibrium constant) */
+    int kind[LBM_CHEM_MAXR];              /* LBM_CHEM_RXN_* */
+    int n_reac[LBM_CHEM_MAXR], reac_sp[LBM_CHEM_MAXR][3];
+    double reac_nu[LBM_CHEM_MAXR][3];
+    int n_prod[LBM_CHEM_MAXR], prod_sp[LBM_CHEM_MAXR][3];
+    double prod_nu[LBM_CHEM_MAXR][3];
+    double A[LBM_CHEM_MAXR], b[LBM_CHEM_MAXR], Ea_R[LBM_CHEM_MAXR];   /* SI: m, mol, s, K */
+    double low[LBM_CHEM_MAXR][3];          /* fall-off low-pressure A, b, Ea/R */
+    double troe[LBM_CHEM_MAXR][4];         /* a, T3, T1, T2 (T2 = 0: term absent) */
+    double eff[LBM_CHEM_MAXR][LBM_CHEM_NSP]; /* third-body efficiencies */
+    /* biomass pyrolysis: r_i = A_i exp(-Ea_R_i / T) rho_B  [kg/(m^3 s)] */
+    double pyro_A[LBM_CHEM_NPYRO], pyro_Ea_R[LBM_CHEM_NPYRO];
+    double pyro_dH[LBM_CHEM_NPYRO];        /* heat absorbed per kg of biomass converted [J/kg] */
+    double pyro_gas[LBM_CHEM_NPYRO][LBM_CHEM_NSP]; /* mol of gas species released per kg of biomass */
+    double biomass_molemass;               /* kg/mol of the biomass unit */
+    double biomass_elements[4];            /* atoms of C, H, O, N per biomass unit */
+    double char_molemass;                  /* kg/mol of char (carbon) */
+    double cs_solid;                       /* heat capacity of biomass and char [J/(kg K)] */
+    /* stiff integrator (two-stage Rosenbrock ROS2, Verwer et al. 1999, adaptive) */
+    double rtol, atol_c, atol_T, atol_B;   /* tolerances: relative; mol/m^3; K; kg/m^3 */
+    int max_steps;                         /* integration fails beyond this many attempted steps */
+} lbm_chem_mech;
+
+/* Surrogate: x_n = (x - in_mean) * in_scale per input, y = y_n * out_scale + out_mean per
+ * output; layers h1 = relu(W1 x_n + b1), h2 = relu(W2 h1 + b2), y_n = W3 h2 + b3 with BF16
+ * weights and layer inputs, FP32 accumulation.  A cell falls back to the plain integrator
+ * when any raw input leaves [hull_lo, hull_hi] or when the element-balance residual of
+ * the prediction exceeds residual_tol (lbm_chem_residual). */
+typedef struct lbm_surrogate {
+    const float* params;                   /* host, LBM_MLP_NPARAM FP32 values */
+    float in_mean[LBM_CHEM_NIN], in_scale[LBM_CHEM_NIN];
+    float out_mean[LBM_CHEM_NOUT], out_scale[LBM_CHEM_NOUT];
+    float hull_lo[LBM_CHEM_NIN], hull_hi[LBM_CHEM_NIN];
+    double residual_tol;
+} lbm_surrogate;
+
+typedef struct lbm_chem lbm_chem;
+
+/* Operator context on one device (mechanism tables, surrogate weights, stream). */
+int lbm_chem_create(const lbm_chem_mech* mech, int device, lbm_chem** out);
+void lbm_chem_destroy(lbm_chem* c);
+const char* lbm_chem_last_error(const lbm_chem* c);
+
+/* [oracle: host arrays] Plain FP64 integration of n cells over dt seconds.
+ * in [16][n], out [14][n]; nsteps [n] (optional) receives the accepted step count of
+ * each cell, or -1 where the integrator hit max_steps (its output is then the state
+ * reached). */
+int lbm_chem_integrate(lbm_chem* c, const double* in, double* out, int* nsteps, long long n, double dt);
+
+/* Load the surrogate (weights are converted to BF16 on the device). */
+int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s);
+
+/* Surrogate evaluation of n cells: FP32 in [16][n] -> out [14][n]; flags [n] (optional):
+ * 0 = prediction kept, 1 = outside the hull, 2 = residual above tolerance.  With
+ * fallback = 1 the flagged cells are integrated by the plain path (their out values
+ * are the plain results); with 0 they keep the raw prediction. */
+int lbm_chem_surrogate(lbm_chem* c, const float* in, float* out, int* flags, long long n, double dt, int fallback);
+
+/* [oracle: host arrays] MLP on normalised data: xn [16][n] -> yn [14][n]. */
+int lbm_mlp_forward(lbm_chem* c, const float* params, const float* xn, float* yn, long long n);
+
+/* [oracle: host arrays] Mean-squared-error loss of the MLP on (xn, yn_target) and its
+ * gradient with respect to all LBM_MLP_NPARAM parameters (summed over cells, divided by
+ * n * 14).  The forward pass rounds weights and layer inputs to BF16 like the inference
+ * kernel; the gradient treats that rounding as the identity. */
+int lbm_mlp_grad(lbm_chem* c, const float* params, const float* xn, const float* yn_target, long long n,
+                 float* grad, double* loss);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, step number t >= 1) on device arrays
+ * of LBM_MLP_NPARAM floats. */
+int lbm_mlp_adam(lbm_chem* c, float* params, const float* grad, float* m, float* v, float lr, int t);
+
+/* [oracle] Element-balance residual of outputs `out` [14] for inputs `in` [16]: the largest
+ * element imbalance between the gas change and what the solid released, over the cell's
+ * total element inventory (gas + biomass).  Host, one cell. */
+double lbm_chem_residual(const lbm_chem_mech* mech, const double* in, const double* out);
+
+/* Attach the operator to a lattice handle (3D or 2D, FP64 or FP32 populations, chemistry on,
+ * the first 12 fluid species = the mechanism's species in order, the first two FIX species =
+ * biomass and char).  After every step the operator maps each cell's fields to its inputs
+ * (T = TS where solid > 0, else TF; p = p0 * rho; rho_B, rho_C = FIX S * rho_solid_ref),
+ * and the next step's macro adds the resulting sources to dS (species: the mass-fraction
+ * change; FIX: the density change / rho_solid_ref; heat: dT / v_scale to TS where solid > 0,
+ * else TF).  mode = LBM_CHEM_OFF detaches.  [oracle] */
+int lbm_chem_attach(lbm_handle* h, lbm_chem* c, int mode, double p0, double rho_solid_ref);
+
+/* Operator statistics since the last attach: cells evaluated, cells that fell back for the
+ * hull and for the residual, and summed device time of the operator's kernels (ms). */
+int lbm_chem_stats(lbm_handle* h, long long* cells, long long* fallback_hull, long long* fallback_residual,
+                   double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_CHEM_H */

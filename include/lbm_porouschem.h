@@ -261,6 +261,7 @@ long long lbm_steps_done(const lbm_handle* h);
 #define LBM_KERNEL_NEE 1  /* boundary extrapolation, all face families */
 #define LBM_KERNEL_FACE 2 /* collision of the face cells */
 #define LBM_KERNEL_STEP 3 /* everything of one time step */
+#define LBM_KERNEL_CHEM 4 /* chemistry operator pass (include/lbm_chem.h), when attached */
 int lbm_profile(lbm_handle* h, int enable);
 int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* launches);
 
@@ -273,7 +274,8 @@ int lbm_state_size(const lbm_handle* h, size_t* bytes);
 int lbm_get_state(lbm_handle* h, void* host, size_t bytes);
 int lbm_set_state(lbm_handle* h, const void* host, size_t bytes);
 
-/* sizeof(lbm_desc) for which = 0, sizeof(lbm_params) for which = 1 (binding self-check). */
+/* sizeof(lbm_desc) for which = 0, sizeof(lbm_params) for 1, and for the chemistry operator
+ * (lbm_chem.h) sizeof(lbm_chem_mech) for 2, sizeof(lbm_surrogate) for 3 (binding self-check). */
 size_t lbm_struct_size(int which);
 
 /* Message of the last error on this handle ("" if none). */

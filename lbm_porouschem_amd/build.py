@@ -32,6 +32,8 @@ UNITS = [
     ("lbm_f64.hip", ["-ffp-contract=off"]),
     ("lbm_f32.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("lbm_f16.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    # chemistry operator: FP64 integrator rounded like the CPU checker; MLP kernels
+    ("chem.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -1,0 +1,306 @@
+"""ctypes binding of the chemistry operator ABI (include/lbm_chem.h) and the
+mechanism / pyrolysis tables it runs on.
+
+`Chem(lib)` wraps one chemistry context of a library exporting the ABI: the HIP
+library (arrays are device tensors, torch as plumbing) or the CPU checker under
+oracle/ (arrays are numpy).  `syngas_mechanism()` builds the default operator
+description: the H2/CO gas-phase sub-mechanism of GRI-Mech 3.0 (12 species, 40
+reactions, generated by make_mechanism.py) plus a three-route biomass pyrolysis.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+NSP = 12
+MAXR = 48
+NPYRO = 3
+NIN = 16
+NOUT = 14
+HIDDEN = 128
+NPARAM = HIDDEN * 16 + HIDDEN + HIDDEN * HIDDEN + HIDDEN + NOUT * HIDDEN + NOUT
+MODE_OFF, MODE_PLAIN, MODE_SURROGATE = 0, 1, 2
+R_GAS = 8.314462618
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MECH_JSON = os.path.join(HERE, "syngas12_gri30.json")
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+class ChemMech(ctypes.Structure):
+    _fields_ = [
+        ("n_species", _i), ("n_reactions", _i), ("molemass", _d * NSP), ("elements", (_d * 4) * NSP),
+        ("nasa_T", (_d * 3) * NSP), ("nasa", ((_d * 7) * 2) * NSP),
+        ("kind", _i * MAXR), ("n_reac", _i * MAXR), ("reac_sp", (_i * 3) * MAXR), ("reac_nu", (_d * 3) * MAXR),
+        ("n_prod", _i * MAXR), ("prod_sp", (_i * 3) * MAXR), ("prod_nu", (_d * 3) * MAXR),
+        ("A", _d * MAXR), ("b", _d * MAXR), ("Ea_R", _d * MAXR), ("low", (_d * 3) * MAXR),
+        ("troe", (_d * 4) * MAXR), ("eff", (_d * NSP) * MAXR),
+        ("pyro_A", _d * NPYRO), ("pyro_Ea_R", _d * NPYRO), ("pyro_dH", _d * NPYRO),
+        ("pyro_gas", (_d * NSP) * NPYRO), ("biomass_molemass", _d), ("biomass_elements", _d * 4),
+        ("char_molemass", _d), ("cs_solid", _d),
+        ("rtol", _d), ("atol_c", _d), ("atol_T", _d), ("atol_B", _d), ("max_steps", _i),
+    ]
+
+
+_f = ctypes.c_float
+
+
+class Surrogate(ctypes.Structure):
+    _fields_ = [
+        ("params", ctypes.POINTER(_f)), ("in_mean", _f * NIN), ("in_scale", _f * NIN),
+        ("out_mean", _f * NOUT), ("out_scale", _f * NOUT), ("hull_lo", _f * NIN), ("hull_hi", _f * NIN),
+        ("residual_tol", _d),
+    ]
+
+
+def load_mechanism_table(path=MECH_JSON):
+    with open(path) as f:
+        return json.load(f)
+
+
+# Biomass pyrolysis of a carbohydrate unit CH2O (30.026 g/mol), three competing first-order
+# routes whose products close the element balance exactly:
+#   1  CH2O -> CO + H2                 (gas)
+#   2  CH2O -> 1/2 CO2 + H2 + 1/2 C    (gas + char)
+#   3  CH2O -> H2O + C                 (dehydration to char)
+# Rates k_i = A_i exp(-Ea_i / (R T)); activation energies 110-150 kJ/mol and log10 A 6.5-8.8,
+# inside the 80-180 kJ/mol and 5-13 ranges of the coupled pyrolysis workload, so the three
+# routes run at 8-20 1/s near 1050 K.  Heats: endothermic gas routes, exothermic charring.
+PYRO_ROUTES = (
+    {"A": 5.8e8, "Ea": 150e3, "dH": 4.18e5, "gas": {"CO": 1.0, "H2": 1.0}},
+    {"A": 3.0e6, "Ea": 110e3, "dH": 2.0e5, "gas": {"CO2": 0.5, "H2": 1.0}},
+    {"A": 1.3e7, "Ea": 125e3, "dH": -5.0e5, "gas": {"H2O": 1.0}},
+)
+BIOMASS_M = 30.026e-3
+CHAR_M = 12.011e-3
+
+
+def syngas_mechanism(routes=PYRO_ROUTES, rtol=1e-6, atol_c=1e-9, atol_T=1e-6, atol_B=1e-9, max_steps=20000,
+                     cs_solid=1500.0, table=None):
+    """ChemMech of the GRI-Mech 3.0 syngas sub-mechanism plus the pyrolysis routes."""
+    t = table or load_mechanism_table()
+    m = ChemMech()
+    sp = t["species"]
+    assert len(sp) == NSP
+    names = [s["name"] for s in sp]
+    m.n_species = NSP
+    m.n_reactions = len(t["reactions"])
+    assert m.n_reactions <= MAXR
+    for k, s in enumerate(sp):
+        m.molemass[k] = s["M"]
+        for e in range(4):
+            m.elements[k][e] = s["elements"][e]
+        for i in range(3):
+            m.nasa_T[k][i] = s["T_ranges"][i]
+        for i in range(7):
+            m.nasa[k][0][i] = s["nasa_low"][i]
+            m.nasa[k][1][i] = s["nasa_high"][i]
+    for r, rec in enumerate(t["reactions"]):
+        m.kind[r] = rec["kind"]
+        m.n_reac[r] = len(rec["reactants"])
+        for i, (j, nu) in enumerate(rec["reactants"]):
+            m.reac_sp[r][i] = j
+            m.reac_nu[r][i] = nu
+        m.n_prod[r] = len(rec["products"])
+        for i, (j, nu) in enumerate(rec["products"]):
+            m.prod_sp[r][i] = j
+            m.prod_nu[r][i] = nu
+        m.A[r], m.b[r], m.Ea_R[r] = rec["A"], rec["b"], rec["Ea_R"]
+        if "low" in rec:
+            for i in range(3):
+                m.low[r][i] = rec["low"][i]
+        if "troe" in rec:
+            for i in range(4):
+                m.troe[r][i] = rec["troe"][i]
+        for j in range(NSP):
+            m.eff[r][j] = rec["efficiencies"][j]
+    for i, route in enumerate(routes):
+        m.pyro_A[i] = route["A"]
+        m.pyro_Ea_R[i] = route["Ea"] / R_GAS
+        m.pyro_dH[i] = route["dH"]
+        for name, nu in route["gas"].items():
+            m.pyro_gas[i][names.index(name)] = nu / BIOMASS_M
+    m.biomass_molemass = BIOMASS_M
+    for e, v in enumerate((1.0, 2.0, 1.0, 0.0)):
+        m.biomass_elements[e] = v
+    m.char_molemass = CHAR_M
+    m.cs_solid = cs_solid
+    m.rtol, m.atol_c, m.atol_T, m.atol_B, m.max_steps = rtol, atol_c, atol_T, atol_B, max_steps
+    return m
+
+
+def species_names(table=None):
+    t = table or load_mechanism_table()
+    return [s["name"] for s in t["species"]]
+
+
+def air_mass_fractions(o2=0.233):
+    """Mass fractions of the 12 species for dry air (O2 + N2)."""
+    y = np.zeros(NSP)
+    names = species_names()
+    y[names.index("O2")] = o2
+    y[names.index("N2")] = 1.0 - o2
+    return y
+
+
+class ChemError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    """Address of a numpy array or a torch tensor."""
+    if isinstance(a, np.ndarray):
+        return ctypes.c_void_p(a.ctypes.data)
+    if a is None:
+        return ctypes.c_void_p(0)
+    return ctypes.c_void_p(a.data_ptr())
+
+
+_CHEM_SYMBOLS = [
+    "lbm_chem_create", "lbm_chem_destroy", "lbm_chem_last_error", "lbm_chem_integrate", "lbm_chem_set_surrogate",
+    "lbm_chem_surrogate", "lbm_mlp_forward", "lbm_mlp_grad", "lbm_mlp_adam", "lbm_chem_residual",
+    "lbm_chem_attach", "lbm_chem_stats",
+]
+
+
+def bind(lib):
+    """Declare the chemistry entry points on a ctypes.CDLL."""
+    vp = ctypes.c_void_p
+    lib.lbm_chem_create.argtypes = [ctypes.POINTER(ChemMech), _i, ctypes.POINTER(vp)]
+    lib.lbm_chem_create.restype = _i
+    lib.lbm_chem_destroy.argtypes = [vp]
+    lib.lbm_chem_destroy.restype = None
+    lib.lbm_chem_last_error.argtypes = [vp]
+    lib.lbm_chem_last_error.restype = ctypes.c_char_p
+    lib.lbm_chem_integrate.argtypes = [vp, vp, vp, vp, ctypes.c_longlong, _d]
+    lib.lbm_chem_integrate.restype = _i
+    lib.lbm_chem_set_surrogate.argtypes = [vp, ctypes.POINTER(Surrogate)]
+    lib.lbm_chem_set_surrogate.restype = _i
+    lib.lbm_chem_surrogate.argtypes = [vp, vp, vp, vp, ctypes.c_longlong, _d, _i]
+    lib.lbm_chem_surrogate.restype = _i
+    lib.lbm_mlp_forward.argtypes = [vp, vp, vp, vp, ctypes.c_longlong]
+    lib.lbm_mlp_forward.restype = _i
+    lib.lbm_mlp_grad.argtypes = [vp, vp, vp, vp, ctypes.c_longlong, vp, ctypes.POINTER(_d)]
+    lib.lbm_mlp_grad.restype = _i
+    lib.lbm_mlp_adam.argtypes = [vp, vp, vp, vp, vp, _f, _i]
+    lib.lbm_mlp_adam.restype = _i
+    lib.lbm_chem_residual.argtypes = [ctypes.POINTER(ChemMech), vp, vp]
+    lib.lbm_chem_residual.restype = _d
+    lib.lbm_chem_attach.argtypes = [vp, vp, _i, _d, _d]
+    lib.lbm_chem_attach.restype = _i
+    lib.lbm_chem_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_d)]
+    lib.lbm_chem_stats.restype = _i
+    return lib
+
+
+class Chem:
+    """One chemistry context of `clib` (an _abi.CLib).  Array arguments are numpy arrays for
+    the CPU checker and torch tensors on the device for the HIP library, structure of
+    arrays: value v of cell i at [v, i]."""
+
+    def __init__(self, clib, mech=None, device=0):
+        self.clib = clib
+        self.lib = bind(clib.lib)
+        self.mech = mech if mech is not None else syngas_mechanism()
+        h = ctypes.c_void_p()
+        rc = self.lib.lbm_chem_create(ctypes.byref(self.mech), device, ctypes.byref(h))
+        if rc != 0:
+            raise ChemError(f"lbm_chem_create failed (status {rc})")
+        self.h = h
+        self._sur = None
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_chem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.lbm_chem_last_error(self.h).decode(errors="replace")
+            raise ChemError(f"{what}: {msg or 'failed'} (status {rc})")
+
+    def integrate(self, inp, out, nsteps=None, dt=1e-5):
+        n = inp.shape[1]
+        self._check(self.lib.lbm_chem_integrate(self.h, _ptr(inp), _ptr(out), _ptr(nsteps), n, dt),
+                    "lbm_chem_integrate")
+
+    def set_surrogate(self, sur):
+        """sur: dict with params (float32 array of NPARAM), in_mean/in_scale (16), out_mean/out_scale
+        (14), hull_lo/hull_hi (16), residual_tol; None unloads."""
+        if sur is None:
+            self._check(self.lib.lbm_chem_set_surrogate(self.h, None), "lbm_chem_set_surrogate")
+            self._sur = None
+            return
+        params = np.ascontiguousarray(sur["params"], dtype=np.float32)
+        assert params.size == NPARAM
+        s = Surrogate()
+        s.params = params.ctypes.data_as(ctypes.POINTER(_f))
+        for name, n in (("in_mean", NIN), ("in_scale", NIN), ("out_mean", NOUT), ("out_scale", NOUT),
+                        ("hull_lo", NIN), ("hull_hi", NIN)):
+            v = np.asarray(sur[name], dtype=np.float32)
+            assert v.size == n, name
+            getattr(s, name)[:] = v.tolist()
+        s.residual_tol = float(sur["residual_tol"])
+        self._check(self.lib.lbm_chem_set_surrogate(self.h, ctypes.byref(s)), "lbm_chem_set_surrogate")
+        self._sur = (params, s)
+
+    def surrogate(self, inp, out, flags=None, dt=1e-5, fallback=1):
+        n = inp.shape[1]
+        self._check(self.lib.lbm_chem_surrogate(self.h, _ptr(inp), _ptr(out), _ptr(flags), n, dt, int(fallback)),
+                    "lbm_chem_surrogate")
+
+    def mlp_forward(self, params, xn, yn):
+        self._check(self.lib.lbm_mlp_forward(self.h, _ptr(params), _ptr(xn), _ptr(yn), xn.shape[1]),
+                    "lbm_mlp_forward")
+
+    def mlp_grad(self, params, xn, yt, grad):
+        loss = _d()
+        self._check(self.lib.lbm_mlp_grad(self.h, _ptr(params), _ptr(xn), _ptr(yt), xn.shape[1], _ptr(grad),
+                                          ctypes.byref(loss)), "lbm_mlp_grad")
+        return loss.value
+
+    def mlp_adam(self, params, grad, m, v, lr, t):
+        self._check(self.lib.lbm_mlp_adam(self.h, _ptr(params), _ptr(grad), _ptr(m), _ptr(v), float(lr), int(t)),
+                    "lbm_mlp_adam")
+
+    def residual(self, inp, out):
+        a = np.ascontiguousarray(inp, dtype=np.float64)
+        b = np.ascontiguousarray(out, dtype=np.float64)
+        return self.lib.lbm_chem_residual(ctypes.byref(self.mech), _ptr(a), _ptr(b))
+
+    def attach(self, handle, mode, p0=101325.0, rho_solid_ref=2.0):
+        rc = self.lib.lbm_chem_attach(handle, self.h, int(mode), float(p0), float(rho_solid_ref))
+        if rc != 0:
+            raise ChemError(f"lbm_chem_attach: {self.clib.lib.lbm_last_error(handle).decode(errors='replace')} "
+                            f"(status {rc})")
+
+    def stats(self, handle):
+        cells, fh, fr = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        ms = _d()
+        rc = self.lib.lbm_chem_stats(handle, ctypes.byref(cells), ctypes.byref(fh), ctypes.byref(fr), ctypes.byref(ms))
+        if rc != 0:
+            raise ChemError(f"lbm_chem_stats failed (status {rc})")
+        return {"cells": cells.value, "fallback_hull": fh.value, "fallback_residual": fr.value, "ms": ms.value}
+
+
+def cell_inputs(T, p, Y, rho_B, rho_C):
+    """Stack operator inputs [16, n] (float64) from per-cell arrays (Y: [12, n])."""
+    T = np.atleast_1d(np.asarray(T, dtype=np.float64))
+    n = T.size
+    x = np.empty((NIN, n))
+    x[0] = T
+    x[1] = p
+    x[2:2 + NSP] = np.asarray(Y, dtype=np.float64).reshape(NSP, -1) if np.ndim(Y) > 1 else np.asarray(Y)[:, None]
+    x[14] = rho_B
+    x[15] = rho_C
+    return x

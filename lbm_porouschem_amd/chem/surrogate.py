@@ -1,0 +1,165 @@
+"""Learned replacement of the plain chemistry integrator: data from the plain path, training
+on the device through the library's own forward/backward kernels, and the surrogate record
+that lbm_chem_set_surrogate loads.
+
+    samples = collect(hip, shape=(96, 64, 64), steps=300, every=20)   # plain FP64 integrator
+    sur, log = train(gchem, *samples)                                  # lbm_mlp_grad + lbm_mlp_adam
+    save(path, sur)
+
+Torch appears here only as device memory for the C ABI's array arguments.
+"""
+import time
+
+import numpy as np
+
+from . import Chem, HIDDEN, MODE_PLAIN, NIN, NOUT, NPARAM
+
+
+def cell_inputs_from_solver(lb, p0=101325.0, rho_solid_ref=2.0):
+    """Operator inputs [16, n] of the solver's cells with solid < 1 (the cells the operator
+    evaluates), built from the fields exactly as the attached operator builds them."""
+    solid = lb.solid.to_numpy().ravel().astype(np.float64)
+    live = solid < 1
+    TF = lb.TF.S.to_numpy().ravel().astype(np.float64)
+    TS = lb.TS.S.to_numpy().ravel().astype(np.float64)
+    T = np.where(solid > 0, TS * lb.TS.v_scale + lb.TS.v_ref, TF * lb.TF.v_scale + lb.TF.v_ref)
+    x = np.empty((NIN, int(live.sum())))
+    x[0] = T[live]
+    x[1] = p0 * lb.rho.to_numpy().ravel().astype(np.float64)[live]
+    for q in range(12):
+        x[2 + q] = lb.species[q].S.to_numpy().ravel().astype(np.float64)[live]
+    x[14] = lb.species[12].S.to_numpy().ravel().astype(np.float64)[live] * rho_solid_ref
+    x[15] = lb.species[13].S.to_numpy().ravel().astype(np.float64)[live] * rho_solid_ref
+    return x
+
+
+def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000, seed=0, precision="fp32",
+            log=print):
+    """Run the coupled pyrolysis case with the plain operator attached and log (inputs, outputs)
+    of the plain FP64 integrator every `every` steps.  Returns (X [16, n], Y [14, n], dt)."""
+    import torch
+
+    from .. import workloads
+    nx, ny, nz = shape
+    lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision)
+    lb.init_simulation()
+    g = Chem(lib)
+    lb.attach_chemistry(g, MODE_PLAIN, rho_solid_ref=lb.chem_rho_solid_ref)
+    xs, ys = [], []
+    t0 = time.perf_counter()
+    for s in range(0, steps + 1, every):
+        if s:
+            lb.run(every)
+        x = cell_inputs_from_solver(lb, rho_solid_ref=lb.chem_rho_solid_ref)
+        xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+        yd = torch.zeros((NOUT, x.shape[1]), dtype=torch.float64, device="cuda")
+        g.integrate(xd, yd, None, dt=lb.dt)
+        xs.append(x)
+        ys.append(yd.cpu().numpy())
+        log(f"collect: step {s}: {x.shape[1]} cells, T {x[0].min():.0f}-{x[0].max():.0f} K "
+            f"({time.perf_counter() - t0:.1f} s)")
+    X = np.concatenate(xs, axis=1)
+    Y = np.concatenate(ys, axis=1)
+    if X.shape[1] > max_samples:
+        idx = np.random.default_rng(seed).choice(X.shape[1], max_samples, replace=False)
+        X, Y = X[:, idx], Y[:, idx]
+    return X, Y, lb.dt
+
+
+def init_params(seed=0):
+    """He-normal weights for the ReLU layers, zero biases."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(NPARAM, dtype=np.float32)
+    o = 0
+    for fan_in, fan_out in ((16, HIDDEN), (HIDDEN, HIDDEN), (HIDDEN, NOUT)):
+        P[o:o + fan_in * fan_out] = rng.standard_normal(fan_in * fan_out) * np.sqrt(2.0 / fan_in)
+        o += fan_in * fan_out + fan_out
+    return P
+
+
+def normalisation(X, Y, margin=0.02):
+    """Per-feature z-score of inputs and outputs; hull = training-input bounding box widened
+    by `margin` of its range."""
+    in_mean = X.mean(axis=1)
+    in_sd = X.std(axis=1)
+    in_sd[in_sd == 0] = 1.0
+    out_mean = Y.mean(axis=1)
+    out_sd = Y.std(axis=1)
+    out_sd[out_sd == 0] = 1.0
+    lo, hi = X.min(axis=1), X.max(axis=1)
+    span = hi - lo
+    return {
+        "in_mean": in_mean.astype(np.float32), "in_scale": (1.0 / in_sd).astype(np.float32),
+        "out_mean": out_mean.astype(np.float32), "out_scale": out_sd.astype(np.float32),
+        "hull_lo": (lo - margin * span).astype(np.float32), "hull_hi": (hi + margin * span).astype(np.float32),
+    }
+
+
+def normalise(X, Y, norm):
+    Xn = ((X.astype(np.float32) - norm["in_mean"][:, None]) * norm["in_scale"][:, None]).astype(np.float32)
+    Yn = ((Y - norm["out_mean"][:, None]) / norm["out_scale"][:, None]).astype(np.float32)
+    return Xn, Yn
+
+
+def train(g, X, Y, epochs=20, batch=8192, lr=2e-3, lr_final=1e-4, holdout=0.05, residual_tol=1e-3, seed=0,
+          params=None, log=print):
+    """Fit the 16-128-128-14 MLP on the device with the library's kernels (lbm_mlp_grad +
+    lbm_mlp_adam), cosine learning-rate decay.  Returns (surrogate dict, history)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n = X.shape[1]
+    perm = rng.permutation(n)
+    nh = max(int(n * holdout), 1)
+    hold, fit = perm[:nh], perm[nh:]
+    norm = normalisation(X[:, fit], Y[:, fit])
+    Xn, Yn = normalise(X, Y, norm)
+    Xd = torch.from_numpy(np.ascontiguousarray(Xn[:, fit])).cuda()
+    Yd = torch.from_numpy(np.ascontiguousarray(Yn[:, fit])).cuda()
+    Xh = torch.from_numpy(np.ascontiguousarray(Xn[:, hold])).cuda()
+    Yh = torch.from_numpy(np.ascontiguousarray(Yn[:, hold])).cuda()
+    P = torch.from_numpy(params if params is not None else init_params(seed)).cuda()
+    G = torch.zeros_like(P)
+    M = torch.zeros_like(P)
+    V = torch.zeros_like(P)
+    nf = Xd.shape[1]
+    steps_per_epoch = max(nf // batch, 1)
+    total = epochs * steps_per_epoch
+    hist = []
+    t = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = torch.randperm(nf, device="cuda")
+        tot = 0.0
+        for b in range(steps_per_epoch):
+            idx = order[b * batch:(b + 1) * batch]
+            xb = Xd[:, idx].contiguous()
+            yb = Yd[:, idx].contiguous()
+            tot += g.mlp_grad(P, xb, yb, G)
+            t += 1
+            rate = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * t / total))
+            g.mlp_adam(P, G, M, V, rate, t)
+        hl = holdout_loss(g, P, Xh, Yh)
+        hist.append({"epoch": ep, "train_loss": tot / steps_per_epoch, "holdout_loss": hl,
+                     "seconds": time.perf_counter() - t0})
+        log(f"train: epoch {ep}: loss {tot / steps_per_epoch:.3e} holdout {hl:.3e} "
+            f"({time.perf_counter() - t0:.1f} s)")
+    sur = dict(norm)
+    sur["params"] = P.cpu().numpy()
+    sur["residual_tol"] = residual_tol
+    return sur, hist
+
+
+def holdout_loss(g, P, Xh, Yh):
+    import torch
+    out = torch.zeros_like(Yh)
+    g.mlp_forward(P, Xh, out)
+    return float(((out - Yh) ** 2).mean())
+
+
+def save(path, sur):
+    np.savez(path, **{k: np.asarray(v) for k, v in sur.items()})
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: (z[k] if z[k].ndim else z[k].item()) for k in z.files}

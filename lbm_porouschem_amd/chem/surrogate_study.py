@@ -1,0 +1,179 @@
+"""Surrogate study of the coupled pyrolysis case with detailed gas-phase kinetics.
+
+    python -m lbm_porouschem_amd.chem.surrogate_study [--train-shape 96,64,64] [--train-steps 300]
+        [--eval-shape 128,64,64] [--eval-steps 400] [--time-shape 256,256,256] [--out DIR]
+
+1. collect: run the case with the plain FP64 operator and log its (inputs, outputs);
+2. train: fit the MLP on the device with the library's forward/backward kernels;
+3. accuracy: run the case from the same initial state with the plain operator and with the
+   surrogate (per-cell fallback on), compare temperature and solid conversion;
+4. timing: wall-clock per coupled step of both paths at the timing size, the operator's own
+   device time, and the surrogate kernel's matrix-core rate.
+Writes study.json and the surrogate (.npz) into --out.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import MODE_PLAIN, MODE_SURROGATE, Chem
+from . import surrogate as S
+
+MLP_FLOP_PER_CELL = 2 * (16 * 128 + 128 * 128 + 128 * 14)  # 40448
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def conversion(lb):
+    """Solid conversion: 1 - biomass mass now / biomass mass at the start (FP64 sums)."""
+    b = lb.species[12].S.to_numpy().astype(np.float64).sum()
+    return b
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_case(lib, shape, steps, mode, sur=None, precision="fp32", sample_every=0):
+    from .. import workloads
+    nx, ny, nz = shape
+    lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision)
+    lb.init_simulation()
+    g = Chem(lib)
+    if sur is not None:
+        g.set_surrogate(sur)
+    lb.attach_chemistry(g, mode, rho_solid_ref=lb.chem_rho_solid_ref)
+    b0 = conversion(lb)
+    lb.sync()
+    t0 = time.perf_counter()
+    lb.run(steps)
+    lb.sync()
+    wall = time.perf_counter() - t0
+    st = lb.chemistry_stats(g)
+    out = {
+        "TF": lb.TF.get_physical_value(lb.TF.S.to_numpy()),
+        "TS": lb.TS.get_physical_value(lb.TS.S.to_numpy()),
+        "biomass0": b0,
+        "biomass": conversion(lb),
+        "char": lb.species[13].S.to_numpy().astype(np.float64).sum(),
+        "species": [lb.species[k].S.to_numpy() for k in range(12)],
+        "wall_s": wall,
+        "stats": st,
+    }
+    return out
+
+
+def accuracy(lib, shape, steps, sur):
+    plain = run_case(lib, shape, steps, MODE_PLAIN)
+    fast = run_case(lib, shape, steps, MODE_SURROGATE, sur=sur)
+    conv_p = 1 - plain["biomass"] / plain["biomass0"]
+    conv_s = 1 - fast["biomass"] / fast["biomass0"]
+    from . import species_names
+    names = species_names()
+    res = {
+        "shape": list(shape), "steps": steps,
+        "rel_l2_TF": rel_l2(fast["TF"], plain["TF"]),
+        "rel_l2_TS": rel_l2(fast["TS"], plain["TS"]),
+        "rel_l2_dTF": rel_l2(fast["TF"] - 1000.0, plain["TF"] - 1000.0),
+        "conversion_plain": conv_p, "conversion_surrogate": conv_s,
+        "rel_err_conversion": abs(conv_s - conv_p) / max(abs(conv_p), 1e-300),
+        "rel_err_char": abs(fast["char"] - plain["char"]) / max(abs(plain["char"]), 1e-300),
+        "rel_l2_species": {n: rel_l2(a, b) for n, a, b in zip(names, fast["species"], plain["species"])},
+        "fallback": fast["stats"],
+        "plain_wall_s": plain["wall_s"], "surrogate_wall_s": fast["wall_s"],
+    }
+    return res
+
+
+def timing(lib, shape, steps_plain, steps_sur, sur):
+    out = {"shape": list(shape)}
+    cells = int(np.prod(shape))
+    for name, mode, steps in (("plain", MODE_PLAIN, steps_plain), ("surrogate", MODE_SURROGATE, steps_sur)):
+        r = run_case(lib, shape, steps, mode, sur=sur if mode == MODE_SURROGATE else None)
+        st = r["stats"]
+        out[name] = {"steps": steps, "ms_per_step": r["wall_s"] * 1e3 / steps,
+                     "chem_ms_per_step": st["ms"] / steps, "cells": st["cells"],
+                     "fallback_hull": st["fallback_hull"], "fallback_residual": st["fallback_residual"],
+                     "MLUPS": cells * steps / r["wall_s"] / 1e6}
+        log(f"timing {name}: {out[name]}")
+    out["speedup_step"] = out["plain"]["ms_per_step"] / out["surrogate"]["ms_per_step"]
+    out["speedup_chem"] = out["plain"]["chem_ms_per_step"] / out["surrogate"]["chem_ms_per_step"]
+    return out
+
+
+def mfma_rate(lib, sur, n=1 << 24, reps=5):
+    """Matrix-core rate of the surrogate kernel alone on n cells (no fallback)."""
+    import torch
+    g = Chem(lib)
+    g.set_surrogate(sur)
+    rng = np.random.default_rng(0)
+    x = np.repeat(((sur["hull_lo"] + sur["hull_hi"]) / 2)[:, None], 4096, axis=1)
+    x = x + (rng.uniform(-0.25, 0.25, x.shape) * (sur["hull_hi"] - sur["hull_lo"])[:, None])
+    xd = torch.from_numpy(np.tile(x.astype(np.float32), (1, n // 4096))).cuda()
+    yd = torch.zeros((14, xd.shape[1]), dtype=torch.float32, device="cuda")
+    g.surrogate(xd, yd, None, fallback=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.surrogate(xd, yd, None, fallback=0)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    tflops = MLP_FLOP_PER_CELL * xd.shape[1] / (ms / 1e3) / 1e12
+    return {"cells": int(xd.shape[1]), "ms": ms, "TFLOPs": tflops, "frac_of_bf16_peak": tflops / BF16_PEAK_TFLOPS}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-shape", default="96,64,64")
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--samples", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--eval-shape", default="128,64,64")
+    ap.add_argument("--eval-steps", type=int, default=400)
+    ap.add_argument("--time-shape", default="256,256,256")
+    ap.add_argument("--time-steps-plain", type=int, default=3)
+    ap.add_argument("--time-steps-surrogate", type=int, default=20)
+    ap.add_argument("--load", default="", help="skip data collection and training: load this surrogate")
+    ap.add_argument("--out", default="gpurun_out/surrogate")
+    args = ap.parse_args()
+    from .. import _abi
+    lib = _abi.load_library()
+    os.makedirs(args.out, exist_ok=True)
+    res = {}
+    if args.load:
+        sur = S.load(args.load)
+    else:
+        shape = tuple(int(v) for v in args.train_shape.split(","))
+        t0 = time.perf_counter()
+        X, Y, dt = S.collect(lib, shape=shape, steps=args.train_steps, every=args.every, max_samples=args.samples,
+                             log=log)
+        res["collect"] = {"shape": list(shape), "steps": args.train_steps, "every": args.every,
+                          "samples": int(X.shape[1]), "seconds": time.perf_counter() - t0}
+        g = Chem(lib)
+        sur, hist = S.train(g, X, Y, epochs=args.epochs, log=log)
+        res["train"] = {"epochs": args.epochs, "history": hist}
+        S.save(os.path.join(args.out, "surrogate.npz"), sur)
+    ev = tuple(int(v) for v in args.eval_shape.split(","))
+    res["accuracy"] = accuracy(lib, ev, args.eval_steps, sur)
+    log(f"accuracy: {json.dumps({k: v for k, v in res['accuracy'].items() if k != 'rel_l2_species'})}")
+    res["mfma"] = mfma_rate(lib, sur)
+    log(f"mfma: {res['mfma']}")
+    if args.time_shape:
+        tshape = tuple(int(v) for v in args.time_shape.split(","))
+        res["timing"] = timing(lib, tshape, args.time_steps_plain, args.time_steps_surrogate, sur)
+    with open(os.path.join(args.out, "study.json"), "w") as f:
+        json.dump(res, f, indent=1, default=float)
+    print(json.dumps({k: res[k] for k in res if k != "train"}, default=float))
+
+
+if __name__ == "__main__":
+    main()

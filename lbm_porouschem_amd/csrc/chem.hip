@@ -1,0 +1,1065 @@
+// chem.hip -- gfx950 kernels and C ABI of the cell-local chemistry operator
+// (include/lbm_chem.h): the plain FP64 stiff integrator, the BF16 matrix-core MLP
+// surrogate with per-cell fallback, the MLP's training kernels, and the pass that a
+// lattice handle runs after every step when the operator is attached.
+//
+// Built with -ffp-contract=off: the FP64 integrator then rounds operation by
+// operation like the CPU checker.
+//
+// Kernels
+//   k_plain<Src>      one thread per cell: gather inputs, two-stage Rosenbrock (ROS2)
+//                     integration over dt in FP64 (finite-difference Jacobian, pivoted
+//                     LU), hand the result to the source policy.
+//   k_surrogate<Src>  one wavefront per 16 cells: inputs staged through LDS, three MLP
+//                     layers as v_mfma_f32_16x16x16_bf16 tiles with the weights in LDS
+//                     (fragment order, conflict-free 8-byte reads), hull and element-
+//                     balance checks, fallback cells appended to a list.
+//   k_plain_list<Src> the plain integrator over the fallback list.
+//   k_train / k_wgrad / k_adam   FP32 training: per-sample forward + backward with
+//                     BF16-rounded operands, per-parameter gradient reduction in FP64,
+//                     Adam update.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/lbm_chem.h"
+#include "chem_iface.hpp"
+
+namespace {
+
+constexpr int NS = LBM_CHEM_NSP;
+constexpr int NR = LBM_CHEM_MAXR;
+constexpr int NP = LBM_CHEM_NPYRO;
+constexpr int NY = NS + 2;  // implicit state: c[12] mol/m^3, T K, rho_B kg/m^3
+constexpr int NI = LBM_CHEM_NIN;
+constexpr int NO = LBM_CHEM_NOUT;
+constexpr int HID = LBM_MLP_HIDDEN;
+constexpr int NPARAM = LBM_MLP_NPARAM;
+constexpr int OFF_W1 = 0, OFF_B1 = HID * NI, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
+              OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + NO * HID;
+constexpr double R_GAS = 8.314462618;
+constexpr double P_ATM = 101325.0;
+constexpr int PLAIN_BLOCK = 128;
+constexpr int SUR_BLOCK = 256;  // 4 wavefronts, 16 cells each per tile
+
+// ---------------------------------------------------------------------------
+// FP64 operator (same algorithm and operation order as the CPU checker)
+// ---------------------------------------------------------------------------
+struct TCache {
+    double kf[NR], k0[NR], lfc[NR], ikc[NR];
+    double u[NS], cv[NS], kp[NP];
+};
+
+__device__ void tcache_fill(const lbm_chem_mech& m, double T, TCache& tc) {
+    const double lnT = log(T), iT = 1.0 / T;
+    double g[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const double* a = T < m.nasa_T[k][1] ? m.nasa[k][0] : m.nasa[k][1];
+        const double cpR = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+        const double hRT = a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] * iT;
+        const double sR = a[0] * lnT + T * (a[1] + T * (a[2] / 2 + T * (a[3] / 3 + T * a[4] / 4))) + a[6];
+        g[k] = hRT - sR;
+        tc.u[k] = (hRT - 1.0) * R_GAS * T;
+        tc.cv[k] = (cpR - 1.0) * R_GAS;
+    }
+    const double RT_P = R_GAS * T / P_ATM;
+    for (int r = 0; r < m.n_reactions; ++r) {
+        tc.kf[r] = m.A[r] * exp(m.b[r] * lnT - m.Ea_R[r] * iT);
+        if (m.kind[r] >= LBM_CHEM_RXN_LINDEMANN) tc.k0[r] = m.low[r][0] * exp(m.low[r][1] * lnT - m.low[r][2] * iT);
+        if (m.kind[r] == LBM_CHEM_RXN_TROE) {
+            const double* t = m.troe[r];
+            double fc = (1.0 - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]);
+            if (t[3] != 0.0) fc += exp(-t[3] * iT);
+            tc.lfc[r] = log10(fc);
+        }
+        double dG = 0.0, dnu = 0.0;
+        for (int i = 0; i < m.n_prod[r]; ++i) {
+            dG += m.prod_nu[r][i] * g[m.prod_sp[r][i]];
+            dnu += m.prod_nu[r][i];
+        }
+        for (int i = 0; i < m.n_reac[r]; ++i) {
+            dG -= m.reac_nu[r][i] * g[m.reac_sp[r][i]];
+            dnu -= m.reac_nu[r][i];
+        }
+        double ikc = exp(dG);
+        const int idn = (int)rint(dnu);
+        for (int i = 0; i < idn; ++i) ikc *= RT_P;
+        for (int i = 0; i < -idn; ++i) ikc /= RT_P;
+        tc.ikc[r] = ikc;
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) tc.kp[i] = m.pyro_A[i] * exp(-m.pyro_Ea_R[i] * iT);
+}
+
+__device__ void rhs(const lbm_chem_mech& m, const TCache& tc, double Csol, const double* y, double* f) {
+    const double* c = y;
+    double w[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) w[k] = 0.0;
+    for (int r = 0; r < m.n_reactions; ++r) {
+        double k = tc.kf[r];
+        if (m.kind[r] != LBM_CHEM_RXN_ELEMENTARY) {
+            double cm = 0.0;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) cm += m.eff[r][j] * c[j];
+            if (m.kind[r] == LBM_CHEM_RXN_THREEBODY) {
+                k *= cm;
+            } else {
+                const double pr = tc.k0[r] * cm / tc.kf[r];
+                double F = 1.0;
+                if (m.kind[r] == LBM_CHEM_RXN_TROE) {
+                    const double lfc = tc.lfc[r];
+                    const double lpr = log10(pr > 1e-300 ? pr : 1e-300);
+                    const double cc = -0.4 - 0.67 * lfc, nn = 0.75 - 1.27 * lfc;
+                    const double f1 = (lpr + cc) / (nn - 0.14 * (lpr + cc));
+                    F = pow(10.0, lfc / (1.0 + f1 * f1));
+                }
+                k *= pr / (1.0 + pr) * F;
+            }
+        }
+        double fw = k, rv = k * tc.ikc[r];
+        for (int i = 0; i < m.n_reac[r]; ++i) {
+            const double ci = c[m.reac_sp[r][i]];
+            for (int t = 0; t < (int)m.reac_nu[r][i]; ++t) fw *= ci;
+        }
+        for (int i = 0; i < m.n_prod[r]; ++i) {
+            const double ci = c[m.prod_sp[r][i]];
+            for (int t = 0; t < (int)m.prod_nu[r][i]; ++t) rv *= ci;
+        }
+        const double q = fw - rv;
+        for (int i = 0; i < m.n_reac[r]; ++i) w[m.reac_sp[r][i]] -= m.reac_nu[r][i] * q;
+        for (int i = 0; i < m.n_prod[r]; ++i) w[m.prod_sp[r][i]] += m.prod_nu[r][i] * q;
+    }
+    double heat = 0.0, cv = Csol;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        heat += tc.u[k] * w[k];
+        cv += c[k] * tc.cv[k];
+    }
+    double rb = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const double ri = tc.kp[i] * y[NS + 1];
+        rb += ri;
+        heat += m.pyro_dH[i] * ri;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) w[k] += m.pyro_gas[i][k] * ri;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) f[k] = w[k];
+    f[NS] = -heat / cv;
+    f[NS + 1] = -rb;
+}
+
+// Integrate one cell: in[16] -> out[14] over dt.  Returns accepted steps or -1.
+__device__ int integrate_cell(const lbm_chem_mech& m, const double* in, double dt, double* out) {
+    const double T0 = in[0], p = in[1], B0 = in[NS + 2], C0 = in[NS + 3];
+    double y[NY], c0[NS];
+    double sym = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sym += in[2 + k] / m.molemass[k];
+    const double rho_g = sym > 0.0 ? p / (R_GAS * T0 * sym) : 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        c0[k] = rho_g * in[2 + k] / m.molemass[k];
+        y[k] = c0[k];
+    }
+    y[NS] = T0;
+    y[NS + 1] = B0;
+#pragma unroll
+    for (int o = 0; o < NO; ++o) out[o] = 0.0;
+    if (!(T0 > 0.0) || !(dt > 0.0)) return -1;
+    const double Csol = (B0 + C0) * m.cs_solid;
+    const double gam = 1.0 + 1.0 / sqrt(2.0);
+
+    double t = 0.0, h = dt;
+    int nacc = 0, natt = 0, rejected = 0, ok = 1;
+    TCache tc;
+    double f0[NY], f1[NY], k1[NY], k2[NY], yn[NY], J[NY][NY];
+    int piv[NY];
+    for (;;) {
+        const int last = h >= dt - t;
+        if (last) h = dt - t;
+        if (natt >= m.max_steps) {
+            ok = 0;
+            break;
+        }
+        ++natt;
+        tcache_fill(m, y[NS], tc);
+        rhs(m, tc, Csol, y, f0);
+        // finite-difference Jacobian, column by column
+        for (int j = 0; j < NY; ++j) {
+            const double yj = y[j];
+            const double floor_j = j < NS ? 1e-6 : (j == NS ? 1.0 : 1e-3);
+            const double a = fabs(yj) > floor_j ? fabs(yj) : floor_j;
+            const double tmp = yj + 1e-7 * a;
+            const double d = tmp - yj;
+            y[j] = tmp;
+            if (j == NS) {
+                TCache tt;
+                tcache_fill(m, tmp, tt);
+                rhs(m, tt, Csol, y, f1);
+            } else {
+                rhs(m, tc, Csol, y, f1);
+            }
+            y[j] = yj;
+            for (int i = 0; i < NY; ++i) J[i][j] = (f1[i] - f0[i]) / d;
+        }
+        for (int i = 0; i < NY; ++i)
+            for (int j = 0; j < NY; ++j) J[i][j] = (i == j ? 1.0 : 0.0) - gam * h * J[i][j];
+        // LU with partial pivoting
+        for (int k = 0; k < NY; ++k) {
+            int pk = k;
+            double best = fabs(J[k][k]);
+            for (int i = k + 1; i < NY; ++i)
+                if (fabs(J[i][k]) > best) {
+                    best = fabs(J[i][k]);
+                    pk = i;
+                }
+            piv[k] = pk;
+            if (pk != k)
+                for (int j = 0; j < NY; ++j) {
+                    const double tt = J[k][j];
+                    J[k][j] = J[pk][j];
+                    J[pk][j] = tt;
+                }
+            if (J[k][k] == 0.0) J[k][k] = 1e-300;
+            const double ip = 1.0 / J[k][k];
+            for (int i = k + 1; i < NY; ++i) {
+                const double l = J[i][k] * ip;
+                J[i][k] = l;
+                for (int j = k + 1; j < NY; ++j) J[i][j] -= l * J[k][j];
+            }
+        }
+        auto solve = [&](double* b) {
+            for (int k = 0; k < NY; ++k)
+                if (piv[k] != k) {
+                    const double tt = b[k];
+                    b[k] = b[piv[k]];
+                    b[piv[k]] = tt;
+                }
+            for (int i = 1; i < NY; ++i)
+                for (int j = 0; j < i; ++j) b[i] -= J[i][j] * b[j];
+            for (int i = NY - 1; i >= 0; --i) {
+                for (int j = i + 1; j < NY; ++j) b[i] -= J[i][j] * b[j];
+                b[i] /= J[i][i];
+            }
+        };
+        for (int i = 0; i < NY; ++i) k1[i] = f0[i];
+        solve(k1);
+        for (int i = 0; i < NY; ++i) yn[i] = y[i] + h * k1[i];
+        tcache_fill(m, yn[NS], tc);
+        rhs(m, tc, Csol, yn, f1);
+        for (int i = 0; i < NY; ++i) k2[i] = f1[i] - 2.0 * k1[i];
+        solve(k2);
+        double err = 0.0;
+        const double atl[3] = {m.atol_c, m.atol_T, m.atol_B};
+        for (int i = 0; i < NY; ++i) {
+            yn[i] = y[i] + 1.5 * h * k1[i] + 0.5 * h * k2[i];
+            const double e = 0.5 * h * (k1[i] + k2[i]);
+            const double ay = fabs(y[i]) > fabs(yn[i]) ? fabs(y[i]) : fabs(yn[i]);
+            const double sc = atl[i < NS ? 0 : i - NS + 1] + m.rtol * ay;
+            err += (e / sc) * (e / sc);
+        }
+        err = sqrt(err / NY);
+        const double es = err > 1e-10 ? err : 1e-10;
+        double fac = 0.9 / sqrt(es);
+        if (err <= 1.0) {
+            for (int i = 0; i < NY; ++i) y[i] = yn[i];
+            ++nacc;
+            if (last) break;
+            t += h;
+            if (fac > 5.0) fac = 5.0;
+            if (fac < 0.2) fac = 0.2;
+            if (rejected && fac > 1.0) fac = 1.0;
+            rejected = 0;
+        } else {
+            if (fac < 0.2) fac = 0.2;
+            if (fac > 1.0) fac = 1.0;
+            rejected = 1;
+        }
+        h *= fac;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) out[k] = (y[k] - c0[k]) * m.molemass[k];
+    out[NS] = y[NS + 1] - B0;
+    out[NS + 1] = y[NS] - T0;
+    return ok ? nacc : -1;
+}
+
+// element-balance residual (lbm_chem_residual)
+__host__ __device__ double residual(const lbm_chem_mech& m, const double* in, const double* out) {
+    const double T = in[0], p = in[1], B = in[NS + 2];
+    double sym = 0.0;
+    for (int k = 0; k < NS; ++k) sym += in[2 + k] / m.molemass[k];
+    const double rho_g = sym > 0.0 && T > 0.0 ? p / (R_GAS * T * sym) : 0.0;
+    double dsum = 0.0;
+    for (int k = 0; k < NS; ++k) dsum += out[k];
+    const double dB = out[NS], dC = -dB - dsum;
+    double inv = 0.0, worst = 0.0;
+    for (int e = 0; e < 4; ++e) {
+        double gas = 0.0, have = 0.0;
+        for (int k = 0; k < NS; ++k) {
+            gas += m.elements[k][e] * out[k] / m.molemass[k];
+            have += m.elements[k][e] * rho_g * in[2 + k] / m.molemass[k];
+        }
+        have += B / m.biomass_molemass * m.biomass_elements[e];
+        double rel = -dB / m.biomass_molemass * m.biomass_elements[e];
+        if (e == 0) rel -= dC / m.char_molemass;
+        inv += have;
+        const double d = fabs(gas - rel);
+        if (d > worst) worst = d;
+    }
+    return inv > 0.0 ? worst / inv : worst;
+}
+
+// ---------------------------------------------------------------------------
+// wave-aggregated list append and counters
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned lane_prefix(unsigned long long bits) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(bits >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bits, 0u));
+}
+
+__device__ __forceinline__ void append(bool want, unsigned idx, unsigned* list, unsigned* count) {
+    const unsigned long long bits = __ballot(want);
+    if (!bits) return;
+    const int leader = __ffsll((long long)bits) - 1;
+    unsigned base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, (unsigned)__popcll(bits));
+    base = __shfl(base, leader, 64);
+    if (want) list[base + lane_prefix(bits)] = idx;
+}
+
+__device__ __forceinline__ void count3(bool act, bool hull, bool res, unsigned long long* stats) {
+    const unsigned long long a = __ballot(act), b = __ballot(hull), c = __ballot(res);
+    if ((threadIdx.x & 63) == 0 && stats) {
+        if (a) atomicAdd(stats + 0, (unsigned long long)__popcll(a));
+        if (b) atomicAdd(stats + 1, (unsigned long long)__popcll(b));
+        if (c) atomicAdd(stats + 2, (unsigned long long)__popcll(c));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// source policies: where inputs come from and where results go
+// ---------------------------------------------------------------------------
+// lbm_chem_integrate: FP64 arrays
+struct SrcF64 {
+    const double* in;
+    double* out;
+    int* nsteps;
+    long long n;
+    __device__ bool load(long long i, double* x) const {
+#pragma unroll
+        for (int v = 0; v < NI; ++v) x[v] = in[(long long)v * n + i];
+        return true;
+    }
+    __device__ void plain(long long i, const double* x, const double* y, int st) const {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) out[(long long)o * n + i] = y[o];
+        if (nsteps) nsteps[i] = st;
+    }
+};
+
+// lbm_chem_surrogate: FP32 arrays, flags, fallback list
+struct SrcF32 {
+    const float* in;
+    float* out;
+    int* flags;
+    long long n;
+    int fallback;
+    unsigned* list;
+    unsigned* count;
+    __device__ bool load(long long i, double* x) const {
+#pragma unroll
+        for (int v = 0; v < NI; ++v) x[v] = (double)in[(long long)v * n + i];
+        return true;
+    }
+    __device__ void predicted(long long i, const double* x, const double* y, int flag) const {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) out[(long long)o * n + i] = (float)y[o];
+        if (flags) flags[i] = flag;
+    }
+    __device__ void plain(long long i, const double* x, const double* y, int) const {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) out[(long long)o * n + i] = (float)y[o];
+    }
+};
+
+// lbm_mlp_forward: normalised FP32 arrays, no checks
+struct SrcNorm {
+    const float* in;
+    float* out;
+    long long n;
+};
+
+// attached lattice handle: cell fields in, lattice-unit sources out
+template <class Real> struct SrcFields {
+    lbmimpl::ChemFields<Real> f;
+    const double* molemass;  // device copy of the mechanism's molar masses
+    __device__ bool load(long long i, double* x) const {
+        const long long N = f.N;
+        const Real sol = f.solid[i];
+        if (!(sol < (Real)1)) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) f.xsrc[(long long)q * N + i] = (Real)0;
+            return false;
+        }
+        x[0] = sol > (Real)0 ? (double)f.TS[i] * f.ts_scale + f.ts_ref : (double)f.TF[i] * f.tf_scale + f.tf_ref;
+        x[1] = f.p0 * (double)f.rho[i];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) x[2 + q] = (double)f.S[(long long)q * N + i];
+        x[14] = (double)f.S[(long long)f.nf * N + i] * f.rho_ref;
+        x[15] = (double)f.S[(long long)(f.nf + 1) * N + i] * f.rho_ref;
+        return true;
+    }
+    __device__ void apply(long long i, const double* x, const double* y) const {
+        const long long N = f.N;
+        double sym = 0.0;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) sym += x[2 + q] / molemass[q];
+        const double rho_g = sym > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
+        double tot = 0.0, dsum = 0.0;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            tot += rho_g * x[2 + q] + y[q];
+            dsum += y[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            f.xsrc[(long long)q * N + i] = (Real)(tot > 0.0 ? (rho_g * x[2 + q] + y[q]) / tot - x[2 + q] : 0.0);
+        f.xsrc[12 * N + i] = (Real)(y[12] / f.rho_ref);
+        f.xsrc[13 * N + i] = (Real)((-y[12] - dsum) / f.rho_ref);
+        const bool sol = f.solid[i] > (Real)0;
+        f.xsrc[14 * N + i] = sol ? (Real)0 : (Real)(y[13] / f.tf_scale);
+        f.xsrc[15 * N + i] = sol ? (Real)(y[13] / f.ts_scale) : (Real)0;
+    }
+    __device__ void predicted(long long i, const double* x, const double* y, int flag) const {
+        if (!flag) apply(i, x, y);
+    }
+    __device__ void plain(long long i, const double* x, const double* y, int) const { apply(i, x, y); }
+};
+
+// ---------------------------------------------------------------------------
+// plain kernels
+// ---------------------------------------------------------------------------
+template <class Src>
+__global__ __launch_bounds__(PLAIN_BLOCK) void k_plain(const lbm_chem_mech* __restrict__ mech, Src src, long long n,
+                                                       double dt, unsigned long long* stats) {
+    const long long i = (long long)blockIdx.x * PLAIN_BLOCK + threadIdx.x;
+    bool act = false;
+    if (i < n) {
+        double x[NI], y[NO];
+        act = src.load(i, x);
+        if (act) {
+            const int st = integrate_cell(*mech, x, dt, y);
+            src.plain(i, x, y, st);
+        }
+    }
+    count3(act, false, false, stats);
+}
+
+template <class Src>
+__global__ __launch_bounds__(PLAIN_BLOCK) void k_plain_list(const lbm_chem_mech* __restrict__ mech, Src src,
+                                                            const unsigned* __restrict__ list,
+                                                            const unsigned* __restrict__ count, double dt) {
+    const unsigned cnt = *count;
+    for (unsigned j = blockIdx.x * PLAIN_BLOCK + threadIdx.x; j < cnt; j += gridDim.x * PLAIN_BLOCK) {
+        const long long i = list[j];
+        double x[NI], y[NO];
+        src.load(i, x);
+        const int st = integrate_cell(*mech, x, dt, y);
+        src.plain(i, x, y, st);
+    }
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// MLP on the matrix cores
+// ---------------------------------------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// weight tiles in fragment order: tile t, lane l, element e at w[(t * 64 + l) * 4 + e]
+// holds A[l % 16][4 * (l / 16) + e] of that tile (16 x 16, rows = neurons)
+constexpr int TILE_W1 = 0;             // 8 row tiles x 1 k tile
+constexpr int TILE_W2 = 8;             // 8 row tiles x 8 k tiles (t = 8 + mt * 8 + kt)
+constexpr int TILE_W3 = 72;            // 1 row tile (outputs 14, 15 zero) x 8 k tiles
+constexpr int NTILE = 80;
+
+struct SurDev {
+    unsigned short w[NTILE * 256];
+    float b1[HID], b2[HID], b3[16];
+    float in_mean[16], in_scale[16], out_mean[16], out_scale[16], lo[16], hi[16];
+    double residual_tol;
+};
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_round(float x) { return __uint_as_float((unsigned)bf16_bits(x) << 16); }
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)bf16_bits(a);
+    r[1] = (short)bf16_bits(b);
+    r[2] = (short)bf16_bits(c);
+    r[3] = (short)bf16_bits(d);
+    return r;
+}
+
+// FP32 parameters -> fragment-ordered BF16 tiles and biases of a SurDev
+__global__ void k_pack(const float* __restrict__ P, SurDev* __restrict__ S) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < NTILE * 256) {
+        const int t = g >> 8, l = (g >> 2) & 63, e = g & 3;
+        const int r = l & 15, k = 4 * (l >> 4) + e;
+        float v = 0.0f;
+        if (t < TILE_W2) {
+            v = P[OFF_W1 + (t * 16 + r) * NI + k];
+        } else if (t < TILE_W3) {
+            const int mt = (t - TILE_W2) >> 3, kt = (t - TILE_W2) & 7;
+            v = P[OFF_W2 + (mt * 16 + r) * HID + kt * 16 + k];
+        } else {
+            const int kt = t - TILE_W3;
+            v = r < NO ? P[OFF_W3 + r * HID + kt * 16 + k] : 0.0f;
+        }
+        S->w[g] = bf16_bits(v);
+    }
+    if (g < HID) {
+        S->b1[g] = P[OFF_B1 + g];
+        S->b2[g] = P[OFF_B2 + g];
+    }
+    if (g < 16) S->b3[g] = g < NO ? P[OFF_B3 + g] : 0.0f;
+}
+
+__device__ __forceinline__ bf16x4 frag(const unsigned short* wl, int t) {
+    return *(const bf16x4*)(wl + ((t * 64 + (int)(threadIdx.x & 63)) << 2));
+}
+
+// One wavefront evaluates the MLP for 16 cells: xin[cell][16] (normalised, LDS) ->
+// yout[cell][16] (normalised outputs, LDS).  Layers are computed transposed
+// (neurons x cells), so each layer's accumulator tile is directly the next layer's
+// B fragment: no LDS round trip between layers.
+__device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __restrict__ bl, const float* xin,
+                         float* yout) {
+    const int l = threadIdx.x & 63, q = l >> 4, cc = l & 15;
+    const float* xr = xin + cc * 16 + 4 * q;
+    const bf16x4 xb = pack4(xr[0], xr[1], xr[2], xr[3]);
+    bf16x4 h1[8], h2[8];
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+        const float* b = bl + mt * 16 + 4 * q;
+        f32x4 acc = {b[0], b[1], b[2], b[3]};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(wl, TILE_W1 + mt), xb, acc, 0, 0, 0);
+        h1[mt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+    }
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+        const float* b = bl + HID + mt * 16 + 4 * q;
+        f32x4 acc = {b[0], b[1], b[2], b[3]};
+#pragma unroll
+        for (int kt = 0; kt < 8; ++kt)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(wl, TILE_W2 + mt * 8 + kt), h1[kt], acc, 0, 0, 0);
+        h2[mt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+    }
+    const float* b = bl + 2 * HID + 4 * q;
+    f32x4 acc = {b[0], b[1], b[2], b[3]};
+#pragma unroll
+    for (int kt = 0; kt < 8; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(wl, TILE_W3 + kt), h2[kt], acc, 0, 0, 0);
+    float* yr = yout + cc * 16 + 4 * q;
+    yr[0] = acc[0];
+    yr[1] = acc[1];
+    yr[2] = acc[2];
+    yr[3] = acc[3];
+}
+
+// Surrogate over n cells.  CHECK = 0: normalised in/out (lbm_mlp_forward).
+template <class Src, int CHECK>
+__global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restrict__ S,
+                                                         const lbm_chem_mech* __restrict__ mech, Src src, long long n,
+                                                         unsigned* list, unsigned* count,
+                                                         unsigned long long* stats) {
+    __shared__ __attribute__((aligned(16))) unsigned short wl[NTILE * 256];
+    __shared__ __attribute__((aligned(16))) float bl[2 * HID + 16];
+    __shared__ __attribute__((aligned(16))) float xs[4][256];
+    __shared__ __attribute__((aligned(16))) float ys[4][256];
+    {
+        const uint4* gw = (const uint4*)S->w;
+        uint4* lw = (uint4*)wl;
+        for (int i = threadIdx.x; i < NTILE * 256 * 2 / 16; i += SUR_BLOCK) lw[i] = gw[i];
+        for (int i = threadIdx.x; i < HID; i += SUR_BLOCK) {
+            bl[i] = S->b1[i];
+            bl[HID + i] = S->b2[i];
+        }
+        if (threadIdx.x < 16) bl[2 * HID + threadIdx.x] = S->b3[threadIdx.x];
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const long long ntile = (n + 63) / 64;
+    for (long long t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const long long cell = t * 64 + wv * 16 + (l & 15);
+        double x[NI];
+        bool act = false;
+        if (l < 16) {
+            if constexpr (CHECK) {
+                if (cell < n) act = src.load(cell, x);
+#pragma unroll
+                for (int v = 0; v < NI; ++v)
+                    xs[wv][l * 16 + v] = act ? ((float)x[v] - S->in_mean[v]) * S->in_scale[v] : 0.0f;
+            } else {
+                act = cell < n;
+#pragma unroll
+                for (int v = 0; v < NI; ++v) xs[wv][l * 16 + v] = act ? src.in[(long long)v * src.n + cell] : 0.0f;
+            }
+        }
+        __syncthreads();
+        mlp_tile(wl, bl, xs[wv], ys[wv]);
+        __syncthreads();
+        bool hull = false, res = false;
+        if (l < 16 && act) {
+            if constexpr (CHECK) {
+                double y[NO];
+#pragma unroll
+                for (int o = 0; o < NO; ++o) y[o] = (double)(ys[wv][l * 16 + o] * S->out_scale[o] + S->out_mean[o]);
+#pragma unroll
+                for (int v = 0; v < NI; ++v)
+                    if (x[v] < (double)S->lo[v] || x[v] > (double)S->hi[v]) hull = true;
+                if (!hull && residual(*mech, x, y) > S->residual_tol) res = true;
+                src.predicted(cell, x, y, hull ? 1 : (res ? 2 : 0));
+            } else {
+#pragma unroll
+                for (int o = 0; o < NO; ++o) src.out[(long long)o * src.n + cell] = ys[wv][l * 16 + o];
+            }
+        }
+        if constexpr (CHECK) {
+            if (list) append(hull || res, (unsigned)cell, list, count);
+            count3(act, hull, res, stats);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training (FP32, BF16-rounded operands like the inference kernel)
+// ---------------------------------------------------------------------------
+// workspace per sample s (sample-major rows): z1[s][128], z2[s][128], dz1[s][128],
+// dz2[s][128], dy[s][16]
+constexpr int WS_ROW = 4 * HID + 16;
+constexpr int TRAIN_BLOCK = 256;
+
+__global__ __launch_bounds__(TRAIN_BLOCK) void k_train(const float* __restrict__ P, const float* __restrict__ xn,
+                                                       const float* __restrict__ yt, long long n,
+                                                       float* __restrict__ ws, double* __restrict__ loss_part) {
+    __shared__ float W2s[HID * HID];
+    __shared__ float W1s[HID * NI];
+    __shared__ float W3s[NO * HID];
+    __shared__ double red[TRAIN_BLOCK];
+    for (int i = threadIdx.x; i < HID * HID; i += TRAIN_BLOCK) W2s[i] = bf16_round(P[OFF_W2 + i]);
+    for (int i = threadIdx.x; i < HID * NI; i += TRAIN_BLOCK) W1s[i] = bf16_round(P[OFF_W1 + i]);
+    for (int i = threadIdx.x; i < NO * HID; i += TRAIN_BLOCK) W3s[i] = bf16_round(P[OFF_W3 + i]);
+    __syncthreads();
+    const long long s = (long long)blockIdx.x * TRAIN_BLOCK + threadIdx.x;
+    double lsum = 0.0;
+    if (s < n) {
+        float* row = ws + s * WS_ROW;
+        float* z1 = row;
+        float* z2 = row + HID;
+        float* dz1 = row + 2 * HID;
+        float* dz2 = row + 3 * HID;
+        float* dy = row + 4 * HID;
+        float xb[NI];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) xb[i] = bf16_round(xn[(long long)i * n + s]);
+        for (int j = 0; j < HID; ++j) {
+            float a = P[OFF_B1 + j];
+#pragma unroll
+            for (int i = 0; i < NI; ++i) a += W1s[j * NI + i] * xb[i];
+            z1[j] = a;
+        }
+        float acc[HID];
+#pragma unroll
+        for (int j = 0; j < HID; ++j) acc[j] = P[OFF_B2 + j];
+        for (int i = 0; i < HID; ++i) {
+            const float h = bf16_round(fmaxf(z1[i], 0.0f));
+#pragma unroll
+            for (int j = 0; j < HID; ++j) acc[j] += W2s[j * HID + i] * h;
+        }
+        float y[NO];
+#pragma unroll
+        for (int o = 0; o < NO; ++o) y[o] = P[OFF_B3 + o];
+#pragma unroll
+        for (int j = 0; j < HID; ++j) {
+            z2[j] = acc[j];
+            const float h = bf16_round(fmaxf(acc[j], 0.0f));
+#pragma unroll
+            for (int o = 0; o < NO; ++o) y[o] += W3s[o * HID + j] * h;
+        }
+        const float norm = (float)(1.0 / ((double)n * NO));
+        float d[NO];
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            const float e = y[o] - yt[(long long)o * n + s];
+            lsum += (double)e * (double)e;
+            d[o] = 2.0f * e * norm;
+            dy[o] = d[o];
+        }
+        // dz2 = (z2 > 0) * W3^T dy, kept in acc
+#pragma unroll
+        for (int j = 0; j < HID; ++j) {
+            float a = 0.0f;
+#pragma unroll
+            for (int o = 0; o < NO; ++o) a += W3s[o * HID + j] * d[o];
+            acc[j] = acc[j] > 0.0f ? a : 0.0f;
+            dz2[j] = acc[j];
+        }
+        for (int i = 0; i < HID; ++i) {
+            float a = 0.0f;
+#pragma unroll
+            for (int j = 0; j < HID; ++j) a += W2s[j * HID + i] * acc[j];
+            dz1[i] = z1[i] > 0.0f ? a : 0.0f;
+        }
+    }
+    red[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int w = TRAIN_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// one thread per parameter: sum over samples in FP64
+__global__ __launch_bounds__(256) void k_wgrad(const float* __restrict__ xn, const float* __restrict__ ws, long long n,
+                                               float* __restrict__ grad) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= NPARAM) return;
+    double a = 0.0;
+    if (p < OFF_B1) {
+        const int j = p / NI, i = p % NI;
+        for (long long s = 0; s < n; ++s) a += (double)ws[s * WS_ROW + 2 * HID + j] * bf16_round(xn[(long long)i * n + s]);
+    } else if (p < OFF_W2) {
+        const int j = p - OFF_B1;
+        for (long long s = 0; s < n; ++s) a += (double)ws[s * WS_ROW + 2 * HID + j];
+    } else if (p < OFF_B2) {
+        const int j = (p - OFF_W2) / HID, i = (p - OFF_W2) % HID;
+        for (long long s = 0; s < n; ++s)
+            a += (double)ws[s * WS_ROW + 3 * HID + j] * bf16_round(fmaxf(ws[s * WS_ROW + i], 0.0f));
+    } else if (p < OFF_W3) {
+        const int j = p - OFF_B2;
+        for (long long s = 0; s < n; ++s) a += (double)ws[s * WS_ROW + 3 * HID + j];
+    } else if (p < OFF_B3) {
+        const int o = (p - OFF_W3) / HID, i = (p - OFF_W3) % HID;
+        for (long long s = 0; s < n; ++s)
+            a += (double)ws[s * WS_ROW + 4 * HID + o] * bf16_round(fmaxf(ws[s * WS_ROW + HID + i], 0.0f));
+    } else {
+        const int o = p - OFF_B3;
+        for (long long s = 0; s < n; ++s) a += (double)ws[s * WS_ROW + 4 * HID + o];
+    }
+    grad[p] = (float)a;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       float lr, float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= NPARAM) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct lbm_chem {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    lbm_chem_mech mech{};
+    lbm_chem_mech* d_mech = nullptr;
+    double* d_molemass = nullptr;
+    SurDev* d_sur = nullptr;    // loaded surrogate
+    SurDev* d_tmp = nullptr;    // scratch for lbm_mlp_forward
+    float* d_params = nullptr;  // staging of host parameters
+    bool have_sur = false;
+    unsigned* d_list = nullptr;
+    unsigned* d_count = nullptr;
+    long long list_cap = 0;
+    float* d_ws = nullptr;
+    long long ws_cap = 0;
+    double* d_loss = nullptr;
+    long long loss_cap = 0;
+    int n_cu = 256;
+    char err[512] = {0};
+};
+
+namespace {
+
+int cerr(lbm_chem* c, int code, const char* fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define CHEM_TRY(expr)                                                                                           \
+    do {                                                                                                         \
+        hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess)                                                                                    \
+            return cerr(c, LBM_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,    \
+                        __LINE__);                                                                               \
+    } while (0)
+
+int ensure_list(lbm_chem* c, long long n) {
+    if (c->list_cap >= n) return LBM_OK;
+    if (c->d_list) CHEM_TRY(hipFree(c->d_list));
+    c->d_list = nullptr;
+    c->list_cap = 0;
+    CHEM_TRY(hipMalloc(&c->d_list, (size_t)(n > 0 ? n : 1) * sizeof(unsigned)));
+    c->list_cap = n;
+    return LBM_OK;
+}
+
+unsigned grid_tiles(const lbm_chem* c, long long n) {
+    const long long tiles = (n + 63) / 64;
+    const long long cap = (long long)c->n_cu * 8;
+    return (unsigned)(tiles < cap ? (tiles > 0 ? tiles : 1) : cap);
+}
+
+unsigned grid_plain(long long n) { return (unsigned)((n + PLAIN_BLOCK - 1) / PLAIN_BLOCK); }
+
+}  // namespace
+
+namespace lbmimpl {
+
+bool chem_has_surrogate(const lbm_chem* c) { return c && c->have_sur; }
+int chem_device(const lbm_chem* c) { return c ? c->device : -1; }
+
+template <class Real> int chem_run_fields(lbm_chem* c, int mode, const ChemFields<Real>& f, hipStream_t stream) {
+    SrcFields<Real> src{f, c->d_molemass};
+    if (f.N <= 0) return LBM_OK;
+    if (mode == LBM_CHEM_PLAIN) {
+        hipLaunchKernelGGL((k_plain<SrcFields<Real>>), dim3(grid_plain(f.N)), dim3(PLAIN_BLOCK), 0, stream, c->d_mech,
+                           src, f.N, f.dt, f.stats);
+    } else {
+        if (!c->have_sur) return cerr(c, LBM_ERR_STATE, "no surrogate loaded");
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, f.count, 1);
+        hipLaunchKernelGGL((k_surrogate<SrcFields<Real>, 1>), dim3(grid_tiles(c, f.N)), dim3(SUR_BLOCK), 0, stream,
+                           c->d_sur, c->d_mech, src, f.N, f.list, f.count, f.stats);
+        hipLaunchKernelGGL((k_plain_list<SrcFields<Real>>), dim3((unsigned)c->n_cu * 4), dim3(PLAIN_BLOCK), 0, stream,
+                           c->d_mech, src, f.list, f.count, f.dt);
+    }
+    CHEM_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
+template int chem_run_fields<float>(lbm_chem*, int, const ChemFields<float>&, hipStream_t);
+template int chem_run_fields<double>(lbm_chem*, int, const ChemFields<double>&, hipStream_t);
+
+}  // namespace lbmimpl
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int lbm_chem_create(const lbm_chem_mech* mech, int device, lbm_chem** out) {
+    if (!mech || !out) return LBM_ERR_INVALID;
+    *out = nullptr;
+    if (mech->n_species != NS || mech->n_reactions < 0 || mech->n_reactions > NR) return LBM_ERR_INVALID;
+    for (int r = 0; r < mech->n_reactions; ++r)
+        if (mech->n_reac[r] < 1 || mech->n_reac[r] > 3 || mech->n_prod[r] < 1 || mech->n_prod[r] > 3)
+            return LBM_ERR_INVALID;
+    lbm_chem* c = new (std::nothrow) lbm_chem();
+    if (!c) return LBM_ERR_OOM;
+    c->device = device;
+    c->mech = *mech;
+    int rc = LBM_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->n_cu = prop.multiProcessorCount;
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
+        if (hipMalloc(&c->d_mech, sizeof(lbm_chem_mech)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMalloc(&c->d_molemass, NS * sizeof(double)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMalloc(&c->d_sur, sizeof(SurDev)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMalloc(&c->d_tmp, sizeof(SurDev)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMalloc(&c->d_params, NPARAM * sizeof(float)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMalloc(&c->d_count, sizeof(unsigned)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMemcpy(c->d_mech, mech, sizeof(lbm_chem_mech), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
+        if (hipMemcpy(c->d_molemass, mech->molemass, NS * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
+    } while (0);
+    if (rc != LBM_OK) {
+        lbm_chem_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+void lbm_chem_destroy(lbm_chem* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    void* ptrs[] = {c->d_mech, c->d_molemass, c->d_sur, c->d_tmp, c->d_params, c->d_list, c->d_count, c->d_ws, c->d_loss};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+const char* lbm_chem_last_error(const lbm_chem* c) { return c ? c->err : "null chemistry context"; }
+
+int lbm_chem_integrate(lbm_chem* c, const double* in, double* out, int* nsteps, long long n, double dt) {
+    if (!c || (!in && n) || (!out && n) || n < 0) return LBM_ERR_INVALID;
+    if (n == 0) return LBM_OK;
+    CHEM_TRY(hipSetDevice(c->device));
+    SrcF64 src{in, out, nsteps, n};
+    hipLaunchKernelGGL((k_plain<SrcF64>), dim3(grid_plain(n)), dim3(PLAIN_BLOCK), 0, c->stream, c->d_mech, src, n, dt,
+                       (unsigned long long*)nullptr);
+    CHEM_TRY(hipGetLastError());
+    CHEM_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
+    if (!c) return LBM_ERR_INVALID;
+    if (!s) {
+        c->have_sur = false;
+        return LBM_OK;
+    }
+    if (!s->params) return cerr(c, LBM_ERR_INVALID, "surrogate without parameters");
+    CHEM_TRY(hipSetDevice(c->device));
+    CHEM_TRY(hipMemcpyAsync(c->d_params, s->params, NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_pack, dim3((NTILE * 256 + 255) / 256), dim3(256), 0, c->stream, c->d_params, c->d_sur);
+    CHEM_TRY(hipGetLastError());
+    // normalisation, hull and tolerance
+    SurDev host_tail;
+    std::memset(&host_tail, 0, sizeof(host_tail));
+    for (int i = 0; i < 16; ++i) {
+        host_tail.in_mean[i] = s->in_mean[i];
+        host_tail.in_scale[i] = s->in_scale[i];
+        host_tail.lo[i] = s->hull_lo[i];
+        host_tail.hi[i] = s->hull_hi[i];
+        host_tail.out_mean[i] = i < NO ? s->out_mean[i] : 0.0f;
+        host_tail.out_scale[i] = i < NO ? s->out_scale[i] : 0.0f;
+    }
+    host_tail.residual_tol = s->residual_tol;
+    const size_t off = offsetof(SurDev, in_mean);
+    CHEM_TRY(hipMemcpyAsync((char*)c->d_sur + off, (const char*)&host_tail + off, sizeof(SurDev) - off,
+                            hipMemcpyHostToDevice, c->stream));
+    CHEM_TRY(hipStreamSynchronize(c->stream));
+    c->have_sur = true;
+    return LBM_OK;
+}
+
+int lbm_chem_surrogate(lbm_chem* c, const float* in, float* out, int* flags, long long n, double dt, int fallback) {
+    if (!c || n < 0 || (n && (!in || !out))) return LBM_ERR_INVALID;
+    if (!c->have_sur) return cerr(c, LBM_ERR_STATE, "no surrogate loaded");
+    if (n > 0xffffffffLL) return cerr(c, LBM_ERR_INVALID, "at most 2^32 - 1 cells per call");
+    if (n == 0) return LBM_OK;
+    CHEM_TRY(hipSetDevice(c->device));
+    int rc = ensure_list(c, n);
+    if (rc) return rc;
+    SrcF32 src{in, out, flags, n, fallback, c->d_list, c->d_count};
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->d_count, 1);
+    hipLaunchKernelGGL((k_surrogate<SrcF32, 1>), dim3(grid_tiles(c, n)), dim3(SUR_BLOCK), 0, c->stream, c->d_sur,
+                       c->d_mech, src, n, fallback ? c->d_list : (unsigned*)nullptr, c->d_count,
+                       (unsigned long long*)nullptr);
+    if (fallback)
+        hipLaunchKernelGGL((k_plain_list<SrcF32>), dim3((unsigned)c->n_cu * 4), dim3(PLAIN_BLOCK), 0, c->stream,
+                           c->d_mech, src, c->d_list, c->d_count, dt);
+    CHEM_TRY(hipGetLastError());
+    CHEM_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_mlp_forward(lbm_chem* c, const float* params, const float* xn, float* yn, long long n) {
+    if (!c || !params || n < 0 || (n && (!xn || !yn))) return LBM_ERR_INVALID;
+    if (n == 0) return LBM_OK;
+    CHEM_TRY(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_pack, dim3((NTILE * 256 + 255) / 256), dim3(256), 0, c->stream, params, c->d_tmp);
+    SrcNorm src{xn, yn, n};
+    hipLaunchKernelGGL((k_surrogate<SrcNorm, 0>), dim3(grid_tiles(c, n)), dim3(SUR_BLOCK), 0, c->stream, c->d_tmp,
+                       c->d_mech, src, n, (unsigned*)nullptr, (unsigned*)nullptr, (unsigned long long*)nullptr);
+    CHEM_TRY(hipGetLastError());
+    CHEM_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_mlp_grad(lbm_chem* c, const float* params, const float* xn, const float* yn_target, long long n, float* grad,
+                 double* loss) {
+    if (!c || !params || !xn || !yn_target || !grad || n <= 0) return LBM_ERR_INVALID;
+    CHEM_TRY(hipSetDevice(c->device));
+    if (c->ws_cap < n) {
+        if (c->d_ws) CHEM_TRY(hipFree(c->d_ws));
+        c->d_ws = nullptr;
+        c->ws_cap = 0;
+        CHEM_TRY(hipMalloc(&c->d_ws, (size_t)n * WS_ROW * sizeof(float)));
+        c->ws_cap = n;
+    }
+    const long long nblk = (n + TRAIN_BLOCK - 1) / TRAIN_BLOCK;
+    if (c->loss_cap < nblk) {
+        if (c->d_loss) CHEM_TRY(hipFree(c->d_loss));
+        c->d_loss = nullptr;
+        c->loss_cap = 0;
+        CHEM_TRY(hipMalloc(&c->d_loss, (size_t)nblk * sizeof(double)));
+        c->loss_cap = nblk;
+    }
+    hipLaunchKernelGGL(k_train, dim3((unsigned)nblk), dim3(TRAIN_BLOCK), 0, c->stream, params, xn, yn_target, n,
+                       c->d_ws, c->d_loss);
+    hipLaunchKernelGGL(k_wgrad, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, xn, c->d_ws, n, grad);
+    CHEM_TRY(hipGetLastError());
+    if (loss) {
+        std::vector<double> part((size_t)nblk);
+        CHEM_TRY(hipMemcpyAsync(part.data(), c->d_loss, (size_t)nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CHEM_TRY(hipStreamSynchronize(c->stream));
+        double L = 0.0;
+        for (double v : part) L += v;
+        *loss = L / ((double)n * NO);
+    } else {
+        CHEM_TRY(hipStreamSynchronize(c->stream));
+    }
+    return LBM_OK;
+}
+
+int lbm_mlp_adam(lbm_chem* c, float* params, const float* grad, float* m, float* v, float lr, int t) {
+    if (!c || !params || !grad || !m || !v || t < 1) return LBM_ERR_INVALID;
+    CHEM_TRY(hipSetDevice(c->device));
+    const float c1 = (float)(1.0 / (1.0 - pow(0.9, t))), c2 = (float)(1.0 / (1.0 - pow(0.999, t)));
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, params, grad, m, v, lr, c1, c2);
+    CHEM_TRY(hipGetLastError());
+    CHEM_TRY(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+double lbm_chem_residual(const lbm_chem_mech* mech, const double* in, const double* out) {
+    if (!mech || !in || !out) return -1.0;
+    return residual(*mech, in, out);
+}
+
+}  // extern "C"

@@ -2,6 +2,7 @@
 // implementations in lbm_f64.hip / lbm_f32.hip (kernels in lbm_impl.hpp, per-cell
 // logic in lbm_core.hpp).
 #include "lbm_impl.hpp"
+#include "../../include/lbm_chem.h"
 
 using namespace lbm;
 using namespace lbmimpl;
@@ -28,7 +29,15 @@ extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
 
-size_t lbm_struct_size(int which) { return which == 0 ? sizeof(lbm_desc) : (which == 1 ? sizeof(lbm_params) : 0); }
+size_t lbm_struct_size(int which) {
+    switch (which) {
+        case 0: return sizeof(lbm_desc);
+        case 1: return sizeof(lbm_params);
+        case 2: return sizeof(lbm_chem_mech);
+        case 3: return sizeof(lbm_surrogate);
+        default: return 0;
+    }
+}
 
 int lbm_create(const lbm_desc* desc, lbm_handle** out) {
     if (!desc || !out) return LBM_ERR_INVALID;
@@ -155,5 +164,16 @@ int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* lau
 }
 
 const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null handle"; }
+
+int lbm_chem_attach(lbm_handle* h, lbm_chem* c, int mode, double p0, double rho_solid_ref) {
+    if (!h) return LBM_ERR_INVALID;
+    if (!h->initialised && mode != LBM_CHEM_OFF) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_chem_attach");
+    return h->impl->chem_attach(c, mode, p0, rho_solid_ref);
+}
+
+int lbm_chem_stats(lbm_handle* h, long long* cells, long long* fallback_hull, long long* fallback_residual, double* ms) {
+    if (!h) return LBM_ERR_INVALID;
+    return h->impl->chem_stats(cells, fallback_hull, fallback_residual, ms);
+}
 
 }  // extern "C"

@@ -217,6 +217,9 @@ template <class Real, class Store> struct Bufs {
     Real* vprof;          // face velocity profiles, 3 values per face cell, faces concatenated
     Real* rprof;          // face density profiles
     long long prof_off[6];
+    // chemistry operator sources of this step's macro, [16][N] (nullptr: operator detached):
+    // 12 fluid species dS, 2 FIX dS, dTF, dTS (lattice units)
+    const Real* xsrc;
 };
 
 // ---------------------------------------------------------------------------
@@ -1228,6 +1231,23 @@ LBM_DEV void side_get_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, 
         if (LBM_MV(chem, P.chem) && j < (M::exact_species ? NF : P.nf)) side_get<L::QS>(B.side, P.nshell, P.slot_sp + j * L::QS, si, p.gP[j]);
 }
 
+// chemistry operator sources (include/lbm_chem.h): added to the reaction sources of the
+// macro, evaluated on the previous step's fields.  Only the 12 + 2 species layout carries it.
+template <class Real, class Store, int NF, int NX, class M = ModelAny>
+LBM_DEV void xsrc_add(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, CellMac<Real, NF, NX>& m) {
+    if constexpr (NF >= 12 && NX >= 2 && !M::exact_species) {
+        if (B.xsrc) {
+            const long long N = P.N;
+#pragma unroll
+            for (int j = 0; j < 12; ++j) m.dS[j] += B.xsrc[(long long)j * N + c];
+            m.dS[NF] += B.xsrc[12 * N + c];
+            m.dS[NF + 1] += B.xsrc[13 * N + c];
+            m.dTF += B.xsrc[14 * N + c];
+            m.dTS += B.xsrc[15 * N + c];
+        }
+    }
+}
+
 // macroscopic fields
 template <class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void mac_put(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, const CellMac<Real, NF, NX>& m) {
@@ -1634,6 +1654,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
             m.rhos = B.rhos[c];
         const Real Tlag = (LBM_MV(chem, P.chem) && LBM_MV(thermal, P.thermal)) ? B.lag[c] : (Real)0;
         macro_scalar<Real, D, NF, NX, M>(P, p, m, Tlag);
+        xsrc_add<Real, Store, NF, NX, M>(P, B, c, m);
         // from here on the face cell carries its post-BC moments
         m.rho = rho_b;
         m.ux = ux_b;
@@ -1701,6 +1722,7 @@ LBM_DEV void bulk_cell(const DevParams<Real>& P, const Bufs<Real, Store>& B, lon
 #pragma unroll
     for (int j = 0; j < NF; ++j) Sold[j] = (LBM_MV(lag_sp, P.lag_sp) && j < (M::exact_species ? NF : P.nf)) ? B.lagS[(long long)j * N + c] : (Real)0;
     macro<Real, D, NF, NX, M>(P, p, m, Tlag, TFold, Sold);
+    xsrc_add<Real, Store, NF, NX, M>(P, B, c, m);
 
     // fast-axis face cells of deep rows: NEE from the neighbouring lane, then proceed
     // like an interior cell (collision from the post-BC state)

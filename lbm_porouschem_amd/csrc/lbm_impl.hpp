@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/lbm_porouschem.h"
+#include "chem_iface.hpp"
 #include "lbm_core.hpp"
 
 using namespace lbm;
@@ -236,6 +237,8 @@ struct Impl {
     virtual int set_state(const void* host, size_t bytes) = 0;
     virtual int profile(int enable) = 0;
     virtual int profile_read(int kernel, double* ms, long long* n) = 0;
+    virtual int chem_attach(lbm_chem* c, int mode, double p0, double rho_ref) = 0;
+    virtual int chem_stats(long long* cells, long long* hull, long long* res, double* ms) = 0;
 };
 
 }  // namespace lbmimpl
@@ -298,8 +301,18 @@ struct ImplT final : Impl {
     bool coupled = false;  // parameters match ModelCoupled: run the specialised bulk kernel
     // device-event timing per kernel class
     bool prof_on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[4];
-    size_t prof_used[4] = {0, 0, 0, 0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[5];
+    size_t prof_used[5] = {0, 0, 0, 0, 0};
+    // chemistry operator (lbm_chem_attach)
+    lbm_chem* chem = nullptr;
+    int chem_mode = 0;
+    double chem_p0 = 0, chem_rho = 0;
+    Real* xsrc_buf = nullptr;
+    unsigned* chem_list = nullptr;
+    unsigned* chem_count = nullptr;
+    unsigned long long* chem_cnt = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> chem_ev;
+    size_t chem_ev_used = 0;
     hipEvent_t ev_begin(int k) {
         if (!prof_on) return nullptr;
         auto& v = prof_ev[k];
@@ -336,11 +349,15 @@ struct ImplT final : Impl {
     }
     ~ImplT() override {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 5; ++k)
             for (auto& e : prof_ev[k]) {
                 (void)hipEventDestroy(e.first);
                 (void)hipEventDestroy(e.second);
             }
+        for (auto& e : chem_ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
         for (void* p : owned) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -802,7 +819,8 @@ struct ImplT final : Impl {
         const lbm_params& p = h_->p;
         for (int s = 0; s < n; ++s) {
             const long long stepno = h_->steps + 1;  // step 1 reads the layout written by the init collision
-            const int wm = s == n - 1;
+            // the chemistry operator reads every cell's fields after each step
+            const int wm = s == n - 1 || chem_mode != 0;
             hipEvent_t e_step = ev_begin(3);
             if (D == 2 && p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW)
                 hipLaunchKernelGGL((k_inflow2d<Real, Store>), dim3(1), dim3(BLOCK), 0, stream, dP, B, (Real)p.flow_bc[0]);
@@ -831,10 +849,110 @@ struct ImplT final : Impl {
                                        dP, B);
             }
             ev_end(2, e_face);
+            if (chem_mode) {
+                int rc = run_chem();
+                if (rc) return rc;
+            }
             ev_end(3, e_step);
             HIP_TRY(hipGetLastError());
             h_->steps++;
         }
+        return LBM_OK;
+    }
+
+    // ---- chemistry operator ----
+    int run_chem() {
+        if (chem_ev_used == chem_ev.size()) {
+            hipEvent_t a, b;
+            HIP_TRY(hipEventCreate(&a));
+            HIP_TRY(hipEventCreate(&b));
+            chem_ev.push_back({a, b});
+        }
+        HIP_TRY(hipEventRecord(chem_ev[chem_ev_used].first, stream));
+        hipEvent_t e4 = ev_begin(4);
+        ChemFields<Real> f;
+        f.N = N;
+        f.TF = B.TF;
+        f.TS = B.TS;
+        f.rho = B.rho;
+        f.S = B.S;
+        f.solid = solid;
+        f.nf = nf;
+        f.tf_scale = h_->p.tf.v_scale;
+        f.tf_ref = h_->p.tf.v_ref;
+        f.ts_scale = h_->p.ts.v_scale;
+        f.ts_ref = h_->p.ts.v_ref;
+        f.p0 = chem_p0;
+        f.rho_ref = chem_rho;
+        f.dt = h_->p.dt;
+        f.xsrc = xsrc_buf;
+        f.list = chem_list;
+        f.count = chem_count;
+        f.stats = chem_cnt;
+        const int rc = chem_run_fields<Real>(chem, chem_mode, f, stream);
+        if (rc) return set_err(h_, rc, "chemistry operator: %s", lbm_chem_last_error(chem));
+        ev_end(4, e4);
+        HIP_TRY(hipEventRecord(chem_ev[chem_ev_used].second, stream));
+        chem_ev_used++;
+        return LBM_OK;
+    }
+
+    int chem_attach(lbm_chem* c, int mode, double p0, double rho_ref) override {
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (mode == LBM_CHEM_OFF) {
+            chem_mode = 0;
+            chem = nullptr;
+            B.xsrc = nullptr;
+            return LBM_OK;
+        }
+        if (!c || (mode != LBM_CHEM_PLAIN && mode != LBM_CHEM_SURROGATE) || !(p0 > 0) || !(rho_ref > 0))
+            return set_err(h_, LBM_ERR_INVALID, "lbm_chem_attach: bad arguments");
+        if (!h_->d.chemical || !h_->d.thermal)
+            return set_err(h_, LBM_ERR_UNSUPPORTED, "chemistry operator needs thermal and chemical modules");
+        if constexpr (NF < 12 || NX < 2 || sizeof(Store) < sizeof(Real)) {
+            return set_err(h_, LBM_ERR_UNSUPPORTED, "chemistry operator needs 12 fluid + 2 FIX species, FP32/FP64");
+        } else {
+            if (nf != LBM_CHEM_NSP || nfix != 2)
+                return set_err(h_, LBM_ERR_UNSUPPORTED, "chemistry operator needs exactly 12 fluid and 2 FIX species");
+            if (chem_device(c) != h_->d.device)
+                return set_err(h_, LBM_ERR_INVALID, "chemistry context lives on another device");
+            if (mode == LBM_CHEM_SURROGATE && !chem_has_surrogate(c))
+                return set_err(h_, LBM_ERR_STATE, "no surrogate loaded");
+            int rc;
+            if (!xsrc_buf) {
+                if ((rc = dalloc(&xsrc_buf, 16 * (size_t)N))) return rc;
+                if ((rc = dalloc(&chem_list, (size_t)N))) return rc;
+                if ((rc = dalloc(&chem_count, 1))) return rc;
+                if ((rc = dalloc(&chem_cnt, 3))) return rc;
+            }
+            // no sources before the first step after attaching
+            HIP_TRY(hipMemsetAsync(xsrc_buf, 0, 16 * (size_t)N * sizeof(Real), stream));
+            HIP_TRY(hipMemsetAsync(chem_cnt, 0, 3 * sizeof(unsigned long long), stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            chem = c;
+            chem_mode = mode;
+            chem_p0 = p0;
+            chem_rho = rho_ref;
+            chem_ev_used = 0;
+            B.xsrc = xsrc_buf;
+            return LBM_OK;
+        }
+    }
+
+    int chem_stats(long long* cells, long long* hull, long long* res, double* ms) override {
+        HIP_TRY(hipStreamSynchronize(stream));
+        unsigned long long c3[3] = {0, 0, 0};
+        if (chem_cnt) HIP_TRY(hipMemcpy(c3, chem_cnt, sizeof(c3), hipMemcpyDeviceToHost));
+        double tot = 0;
+        for (size_t i = 0; i < chem_ev_used; ++i) {
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, chem_ev[i].first, chem_ev[i].second));
+            tot += t;
+        }
+        if (cells) *cells = (long long)c3[0];
+        if (hull) *hull = (long long)c3[1];
+        if (res) *res = (long long)c3[2];
+        if (ms) *ms = tot;
         return LBM_OK;
     }
 
@@ -984,12 +1102,12 @@ struct ImplT final : Impl {
         HIP_TRY(hipStreamSynchronize(stream));
         prof_on = enable != 0;
         if (prof_on)
-            for (int k = 0; k < 4; ++k) prof_used[k] = 0;
+            for (int k = 0; k < 5; ++k) prof_used[k] = 0;
         return LBM_OK;
     }
 
     int profile_read(int kernel, double* ms, long long* n) override {
-        if (kernel < 0 || kernel > 3) return set_err(h_, LBM_ERR_INVALID, "kernel class %d", kernel);
+        if (kernel < 0 || kernel > 4) return set_err(h_, LBM_ERR_INVALID, "kernel class %d", kernel);
         HIP_TRY(hipStreamSynchronize(stream));
         double tot = 0;
         for (size_t i = 0; i < prof_used[kernel]; ++i) {
@@ -1015,6 +1133,9 @@ Impl* make_for_species(lbm_handle* h, int nf, int nfix) {
     if (nf == 0 && nfix == 0) return new ImplT<Real, Store, D, 0, 0>(h);
     if (nf <= 4 && nfix <= 2) return new ImplT<Real, Store, D, 4, 2>(h);
     if (nf <= 8 && nfix <= 4) return new ImplT<Real, Store, D, 8, 4>(h);
+    // the layout of the chemistry operator (12 transported gas species, biomass, char)
+    if constexpr (sizeof(Store) == sizeof(Real))
+        if (nf <= 12 && nfix <= 2) return new ImplT<Real, Store, D, 12, 2>(h);
     return nullptr;
 }
 

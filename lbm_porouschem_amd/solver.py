@@ -989,6 +989,21 @@ class LBMSolverBase:
             self._pending += 1
         self.tLattice += 1
 
+    def attach_chemistry(self, chem, mode, p0=101325.0, rho_solid_ref=2.0):
+        """Attach a chemistry operator context (lbm_porouschem_amd.chem.Chem) to this solver:
+        mode 1 = plain FP64 stiff integration, 2 = MLP surrogate with per-cell fallback,
+        0 = detach.  Needs 12 fluid species (the mechanism's, in its order) and 2 FIX species
+        (biomass, char); call after init_simulation()."""
+        if self._handle is None:
+            raise RuntimeError("init_simulation() must be called before attach_chemistry()")
+        self._flush()
+        chem.attach(self._handle, mode, p0, rho_solid_ref)
+
+    def chemistry_stats(self, chem):
+        """Cells evaluated, hull / residual fallbacks and device time of the attached operator."""
+        self._flush()
+        return chem.stats(self._handle)
+
     def run(self, nsteps):
         """Advance nsteps in one batch of launches."""
         for _ in range(int(nsteps)):

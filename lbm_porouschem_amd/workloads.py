@@ -166,6 +166,73 @@ def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", col
     return lb
 
 
+def pyrolysis12(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", collision="mrt", T0=1000.0,
+                T_in=1200.0, rho_solid_ref=2.0):
+    """3D coupled case with detailed gas-phase kinetics: D3Q19 flow, D3Q7 fluid/solid temperature,
+    the 12 transported gas species of the GRI-Mech 3.0 H2/CO sub-mechanism (chem.species_names())
+    as D3Q7 lattices, FIX biomass and char.  Gray spheres (radius 6-12 lu, solid 0.5 inside)
+    holding biomass density 1 (x rho_solid_ref kg/m^3); air everywhere at T0; hot-air inlet at
+    T_in.  The chemistry runs in the attached operator (lbm_porouschem_amd.chem), so the
+    solver's own reaction list is empty.  dx = 1e-4 m, dt = 1e-5 s: hot-gas viscosity 1.4e-4
+    m^2/s gives tau = 0.92, species diffusivity 2e-4 m^2/s gives tau = 1.3."""
+    from . import chem as chemmod
+    from .solver import LBM3DSolver
+    dx, dt = 1e-4, 1e-5
+    lb = LBM3DSolver(nx * dx, ny * dx, nz * dx, dx=dx, dt=dt, name="pyrolysis12", isThermal=True, isChemical=True,
+                     isPoro=True, isRadiation=False, precision=precision, collision=collision, lib=lib)
+    assert (lb.nx, lb.ny, lb.nz) == (nx, ny, nz), (lb.nx, lb.ny, lb.nz)
+    lb.source_term_model = SOURCE_TERM.MICRO
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(1.4e-4)
+    if collision == "mrt":
+        lb.set_mrt_standard_rates()
+    solid, mask = gray_packing((nx, ny, nz), 6.0, 12.0, 0.5, seed, 0.5)
+    lb.add_solid(solid)
+    lb.set_poro_Darcy(mask * 1.0, 2e6)
+    lb.set_heat_exchange_surface(mask * 1.0)
+    lb.set_heat_exchange_coef(mask * 5e3)
+    Trange = [300.0, 1300.0]
+    lb.set_fluid_Trange(Trange)
+    lb.set_solid_Trange(Trange)
+    lb.set_fluid_Prandtl(0.71)
+    lb.set_fluid_capacity(1100.0)
+    lb.set_solid_thermal_diff(2e-6)
+    lb.set_solid_capacity(1500.0)
+    table = chemmod.load_mechanism_table()
+    names = [sp["name"] for sp in table["species"]]
+    for sp in table["species"]:
+        lb.set_specie(sp["name"], False, sp["M"] * 1e3)
+        lb.set_specie_diff(sp["name"], 2e-4)
+        lb.set_specie_enthalpy(sp["name"], 0.0)
+        lb.set_specie_capacity(sp["name"], 30.0)
+    lb.set_specie("biomass", True, chemmod.BIOMASS_M * 1e3)
+    lb.set_specie("char", True, chemmod.CHAR_M * 1e3)
+    air = chemmod.air_mass_fractions()
+    for k, name in enumerate(names):
+        lb.init_specie(name, float(air[k]))
+    lb.init_specie("biomass", mask * 1.0)
+    lb.set_BC(0, BC_FLOW.inlet)
+    lb.set_v_BC_value(0, [0.02, 0.0, 0.0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    for f in range(2, 6):
+        lb.set_BC(f, BC_FLOW.wall)
+    lb.set_TF_BC(0, BC.fixedValue)
+    lb.set_TF_BC_value(0, T_in)
+    lb.set_TS_BC(0, BC.fixedValue)
+    lb.set_TS_BC_value(0, T_in)
+    lb.set_TF_BC(1, BC.zeroGradient)
+    lb.set_species_BC(1, BC.zeroGradient)
+    for k, name in enumerate(names):
+        lb.set_specie_BC(name, 0, BC.fixedValue)
+        lb.set_specie_BC_value(name, 0, float(air[k]))
+    lb.init_field(lb.rho, 1.0)
+    lb.init_field(lb.TF.S, float(lb.TF.get_normalized_value(T0)))
+    lb.init_field(lb.TS.S, float(lb.TS.get_normalized_value(T0)))
+    lb.chem_rho_solid_ref = rho_solid_ref
+    return lb
+
+
 # algorithmic HBM bytes per cell-step (populations read once and written once,
 # plus irreducible per-cell auxiliary reads / read-writes)
 def bytes_per_cell(q_tot, pop_bytes, n_ro, n_rw, aux_bytes):
@@ -180,3 +247,5 @@ COUPLED3D_BYTES = {
     "fp16": bytes_per_cell(COUPLED3D_Q, 2, 3, 3, 4),  # 280: FP16 populations, FP32 aux
 }
 PLAIN2D_BYTES_FP64 = bytes_per_cell(9 + 5 + 5, 8, 3, 0, 8)  # 328
+PYRO12_Q = 19 + 7 + 7 + 12 * 7  # 117
+PYRO12_BYTES_FP32 = bytes_per_cell(PYRO12_Q, 4, 3, 3, 4)  # 972

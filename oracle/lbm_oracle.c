@@ -40,6 +40,7 @@
 #include <stdarg.h>
 
 #include "../include/lbm_porouschem.h"
+#include "chem_oracle.h"
 
 #ifndef ORACLE_REAL
 #define ORACLE_REAL double
@@ -84,6 +85,13 @@ struct lbm_handle {
     real* vprof[6];
     real* rprof[6];
     size_t face_n[6];
+    /* chemistry operator (lbm_chem_attach): sources of the next macro, [16][N] */
+    lbm_chem* chem;
+    int chem_mode;
+    double chem_p0, chem_rho_ref;
+    int chem_sp[LBM_CHEM_NSP], chem_fix[2];
+    double* xsrc;
+    long long chem_cells, chem_fb_hull, chem_fb_res;
 };
 
 static int set_err(lbm_handle* h, int code, const char* fmt, ...) {
@@ -658,6 +666,15 @@ static void update_dS(lbm_handle* h, size_t i) {
         else
             h->TF.dS[i] += acc[ns] / tf_capacity(h, i) / h->rho[i];
     }
+    if (h->chem_mode) {
+        /* chemistry operator sources evaluated on the previous step's fields */
+        const size_t N = h->N;
+        for (int q = 0; q < LBM_CHEM_NSP; ++q) h->sp[h->chem_sp[q]].dS[i] += (real)h->xsrc[q * N + i];
+        h->sp[h->chem_fix[0]].dS[i] += (real)h->xsrc[12 * N + i];
+        h->sp[h->chem_fix[1]].dS[i] += (real)h->xsrc[13 * N + i];
+        h->TF.dS[i] += (real)h->xsrc[14 * N + i];
+        h->TS.dS[i] += (real)h->xsrc[15 * N + i];
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1050,6 +1067,7 @@ void lbm_destroy(lbm_handle* h) {
         free(h->sp[j].S); free(h->sp[j].dS); free(h->sp[j].g); free(h->sp[j].G);
     }
     for (int fce = 0; fce < 6; ++fce) { free(h->vprof[fce]); free(h->rprof[fce]); }
+    free(h->xsrc);
     free(h);
 }
 
@@ -1200,6 +1218,107 @@ static int check_div(lbm_handle* h) {
     return LBM_OK;
 }
 
+/* ------------------------------------------------------------------ */
+/* chemistry operator coupling (include/lbm_chem.h)                     */
+/* ------------------------------------------------------------------ */
+/* Cell fields after a completed step -> operator inputs -> sources of the next macro.
+ * Species: change of the mass fraction after adding the partial-density change;
+ * FIX: density change / rho_solid_ref (char by mass conservation); heat: dT / v_scale
+ * to TS where solid > 0, else TF.  Cells with solid >= 1 carry no gas and get none. */
+static void chem_operator(lbm_handle* h) {
+    const size_t N = h->N;
+    const lbm_chem_mech* m = &h->chem->mech;
+    long long fb_h = 0, fb_r = 0, cells = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : fb_h, fb_r, cells)
+    for (long long ii = 0; ii < (long long)N; ++ii) {
+        const size_t i = (size_t)ii;
+        double in[LBM_CHEM_NIN], out[LBM_CHEM_NOUT];
+        for (int q = 0; q < 16; ++q) h->xsrc[q * N + i] = 0.0;
+        if (!(h->solid[i] < 1)) continue;
+        const int sol = h->solid[i] > 0;
+        in[0] = sol ? (double)h->TS.S[i] * h->p.ts.v_scale + h->p.ts.v_ref
+                    : (double)h->TF.S[i] * h->p.tf.v_scale + h->p.tf.v_ref;
+        in[1] = h->chem_p0 * (double)h->rho[i];
+        for (int q = 0; q < LBM_CHEM_NSP; ++q) in[2 + q] = (double)h->sp[h->chem_sp[q]].S[i];
+        in[14] = (double)h->sp[h->chem_fix[0]].S[i] * h->chem_rho_ref;
+        in[15] = (double)h->sp[h->chem_fix[1]].S[i] * h->chem_rho_ref;
+        if (h->chem_mode == LBM_CHEM_SURROGATE) {
+            const int f = chem_oracle_surrogate_cell(h->chem, in, h->p.dt, 1, out);
+            fb_h += f == 1;
+            fb_r += f == 2;
+        } else {
+            chem_oracle_cell(m, in, h->p.dt, out);
+        }
+        cells += 1;
+        double sym = 0.0;
+        for (int q = 0; q < LBM_CHEM_NSP; ++q) sym += in[2 + q] / m->molemass[q];
+        const double rho_g = sym > 0.0 ? in[1] / (8.314462618 * in[0] * sym) : 0.0;
+        double tot = 0.0, dsum = 0.0;
+        for (int q = 0; q < LBM_CHEM_NSP; ++q) {
+            tot += rho_g * in[2 + q] + out[q];
+            dsum += out[q];
+        }
+        for (int q = 0; q < LBM_CHEM_NSP; ++q)
+            h->xsrc[q * N + i] = tot > 0.0 ? (rho_g * in[2 + q] + out[q]) / tot - in[2 + q] : 0.0;
+        h->xsrc[12 * N + i] = out[12] / h->chem_rho_ref;
+        h->xsrc[13 * N + i] = (-out[12] - dsum) / h->chem_rho_ref;
+        if (sol)
+            h->xsrc[15 * N + i] = out[13] / h->p.ts.v_scale;
+        else
+            h->xsrc[14 * N + i] = out[13] / h->p.tf.v_scale;
+    }
+    h->chem_cells += cells;
+    h->chem_fb_hull += fb_h;
+    h->chem_fb_res += fb_r;
+}
+
+int lbm_chem_attach(lbm_handle* h, lbm_chem* c, int mode, double p0, double rho_solid_ref) {
+    if (!h) return LBM_ERR_INVALID;
+    if (mode == LBM_CHEM_OFF) {
+        h->chem_mode = 0;
+        h->chem = NULL;
+        return LBM_OK;
+    }
+    if (!h->initialised) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_chem_attach");
+    if (!c || (mode != LBM_CHEM_PLAIN && mode != LBM_CHEM_SURROGATE) || !(p0 > 0) || !(rho_solid_ref > 0))
+        return set_err(h, LBM_ERR_INVALID, "lbm_chem_attach: bad arguments");
+    if (!h->d.chemical || !h->d.thermal) return set_err(h, LBM_ERR_UNSUPPORTED, "chemistry operator needs thermal and chemical modules");
+    if (mode == LBM_CHEM_SURROGATE && !c->have_surrogate) return set_err(h, LBM_ERR_STATE, "no surrogate loaded");
+    int nf = 0, nx = 0;
+    for (int j = 0; j < h->d.n_species; ++j) {
+        if (is_fix(h, j)) {
+            if (nx < 2) h->chem_fix[nx] = j;
+            ++nx;
+        } else {
+            if (nf < LBM_CHEM_NSP) h->chem_sp[nf] = j;
+            ++nf;
+        }
+    }
+    if (nf != LBM_CHEM_NSP || nx != 2)
+        return set_err(h, LBM_ERR_UNSUPPORTED, "chemistry operator needs exactly 12 fluid and 2 FIX species");
+    if (!h->xsrc) {
+        h->xsrc = (double*)calloc(16 * h->N, sizeof(double));
+        if (!h->xsrc) return set_err(h, LBM_ERR_OOM, "xsrc");
+    } else {
+        memset(h->xsrc, 0, 16 * h->N * sizeof(double));
+    }
+    h->chem = c;
+    h->chem_mode = mode;
+    h->chem_p0 = p0;
+    h->chem_rho_ref = rho_solid_ref;
+    h->chem_cells = h->chem_fb_hull = h->chem_fb_res = 0;
+    return LBM_OK;
+}
+
+int lbm_chem_stats(lbm_handle* h, long long* cells, long long* fallback_hull, long long* fallback_residual, double* ms) {
+    if (!h) return LBM_ERR_INVALID;
+    if (cells) *cells = h->chem_cells;
+    if (fallback_hull) *fallback_hull = h->chem_fb_hull;
+    if (fallback_residual) *fallback_residual = h->chem_fb_res;
+    if (ms) *ms = 0.0;
+    return LBM_OK;
+}
+
 int lbm_step(lbm_handle* h, int nsteps) {
     if (!h) return LBM_ERR_INVALID;
     if (!h->initialised) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_step");
@@ -1209,6 +1328,7 @@ int lbm_step(lbm_handle* h, int nsteps) {
         collision_source_streaming(h);
         macro(h);
         if (h->p.bc_model == LBM_BCMODEL_NEE) boundary_nee(h);
+        if (h->chem_mode) chem_operator(h);
         h->steps++;
     }
     return check_div(h);
@@ -1387,5 +1507,13 @@ int lbm_oracle_lattice(lbm_handle* h, int scalar, int* e, double* w, int* lr) {
     }
     return Q;
 }
-size_t lbm_struct_size(int which) { return which == 0 ? sizeof(lbm_desc) : (which == 1 ? sizeof(lbm_params) : 0); }
+size_t lbm_struct_size(int which) {
+    switch (which) {
+        case 0: return sizeof(lbm_desc);
+        case 1: return sizeof(lbm_params);
+        case 2: return sizeof(lbm_chem_mech);
+        case 3: return sizeof(lbm_surrogate);
+        default: return 0;
+    }
+}
 const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null handle"; }

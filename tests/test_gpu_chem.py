@@ -1,0 +1,226 @@
+"""Parity of the HIP chemistry operator (lbm_porouschem_amd/csrc/chem.hip) with the CPU
+checker (oracle/chem_oracle.c): FP64 integrator, BF16 matrix-core MLP forward, training
+gradient and Adam update, surrogate checks with fallback, and the coupled step with the
+operator attached.  All calls go through the C ABI of include/lbm_chem.h."""
+import numpy as np
+import pytest
+
+from lbm_porouschem_amd import chem, workloads
+
+pytestmark = pytest.mark.gpu
+
+P_ATM = 101325.0
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def random_states(n, seed):
+    rng = np.random.default_rng(seed)
+    names = chem.species_names()
+    Y = np.zeros((12, n))
+    Y[names.index("O2")] = rng.uniform(0.05, 0.233, n)
+    Y[names.index("CO")] = rng.uniform(0.0, 0.06, n)
+    Y[names.index("H2")] = rng.uniform(0.0, 0.01, n)
+    Y[names.index("H2O")] = rng.uniform(0.0, 0.05, n)
+    Y[names.index("CO2")] = rng.uniform(0.0, 0.05, n)
+    Y[names.index("OH")] = rng.uniform(0.0, 1e-5, n)
+    Y[names.index("H")] = rng.uniform(0.0, 1e-6, n)
+    Y[names.index("N2")] = 1.0 - Y.sum(0)
+    T = rng.uniform(900.0, 1350.0, n)
+    B = rng.uniform(0.0, 2.0, n) * (rng.uniform(size=n) < 0.6)
+    C = rng.uniform(0.0, 0.3, n)
+    p = P_ATM * rng.uniform(0.98, 1.02, n)
+    return chem.cell_inputs(T, p, Y, B, C)
+
+
+@pytest.fixture(scope="module")
+def gchem(hip):
+    return chem.Chem(hip)
+
+
+@pytest.fixture(scope="module")
+def ochem(oracle64):
+    return chem.Chem(oracle64)
+
+
+def test_integrate_matches_checker(gchem, ochem):
+    n = 200
+    x = random_states(n, 11)
+    ref = np.zeros((14, n))
+    rs = np.zeros(n, dtype=np.int32)
+    ochem.integrate(x, ref, rs, dt=1e-5)
+    out = dev(np.zeros((14, n)))
+    ns = dev(np.zeros(n, dtype=np.int32))
+    gchem.integrate(dev(x), out, ns, dt=1e-5)
+    got, gs = host(out), host(ns)
+    assert np.all(rs > 0) and np.all(gs > 0)
+    # same algorithm and operation order in FP64: identical step sequences, results equal up to
+    # the last-bit differences of the device and host exp/log/pow
+    same = gs == rs
+    assert same.mean() >= 0.95, (gs[~same], rs[~same])
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-300
+    d = np.abs(got - ref) / scale
+    assert d[:, same].max() <= 1e-9, d.max()
+    assert d.max() <= 1e-4
+
+
+def test_integrate_empty_and_tail(gchem, ochem):
+    gchem.integrate(dev(np.zeros((16, 0))), dev(np.zeros((14, 0))), None, dt=1e-5)
+    x = random_states(1, 12)
+    ref = np.zeros((14, 1))
+    ochem.integrate(x, ref, None, dt=1e-5)
+    out = dev(np.zeros((14, 1)))
+    gchem.integrate(dev(x), out, None, dt=1e-5)
+    assert np.allclose(host(out), ref, rtol=1e-9, atol=1e-300)
+
+
+def random_params(seed, scale=0.15):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(chem.NPARAM) * scale).astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000])
+def test_mfma_forward_matches_checker(gchem, ochem, n):
+    """The BF16 v_mfma tile layout: any transposed operand or misplaced lane shows up here."""
+    P = random_params(21)
+    xn = np.random.default_rng(n).standard_normal((16, n)).astype(np.float32)
+    ref = np.zeros((14, n), dtype=np.float32)
+    ochem.mlp_forward(P, xn, ref)
+    out = dev(np.zeros((14, n), dtype=np.float32))
+    gchem.mlp_forward(dev(P), dev(xn), out)
+    got = host(out)
+    assert np.linalg.norm(got - ref) <= 2e-3 * np.linalg.norm(ref)
+    assert np.max(np.abs(got - ref)) <= 2e-2 * np.max(np.abs(ref))
+
+
+def test_training_gradient_matches_checker(gchem, ochem):
+    n = 777
+    rng = np.random.default_rng(7)
+    P = random_params(22)
+    xn = rng.standard_normal((16, n)).astype(np.float32)
+    yt = rng.standard_normal((14, n)).astype(np.float32)
+    g_ref = np.zeros(chem.NPARAM, dtype=np.float32)
+    l_ref = ochem.mlp_grad(P, xn, yt, g_ref)
+    g = dev(np.zeros(chem.NPARAM, dtype=np.float32))
+    l = gchem.mlp_grad(dev(P), dev(xn), dev(yt), g)
+    assert l == pytest.approx(l_ref, rel=1e-5)
+    assert np.linalg.norm(host(g) - g_ref) <= 1e-4 * np.linalg.norm(g_ref)
+
+
+def test_adam_matches_checker(gchem, ochem):
+    rng = np.random.default_rng(8)
+    P = rng.standard_normal(chem.NPARAM).astype(np.float32)
+    Pd, md, vd = dev(P.copy()), dev(np.zeros_like(P)), dev(np.zeros_like(P))
+    m, v = np.zeros_like(P), np.zeros_like(P)
+    for t in range(1, 4):
+        g = rng.standard_normal(chem.NPARAM).astype(np.float32)
+        ochem.mlp_adam(P, g, m, v, 2e-3, t)
+        gchem.mlp_adam(Pd, dev(g), md, vd, 2e-3, t)
+    np.testing.assert_array_equal(host(Pd), P)
+
+
+def toy_surrogate(x, seed=31):
+    """A random network whose outputs are scaled to the size of real increments, with the
+    hull set from half of the inputs' spread so that some cells fall outside."""
+    lo = np.percentile(x, 10, axis=1).astype(np.float32)
+    hi = np.percentile(x, 95, axis=1).astype(np.float32)
+    mean = x.mean(axis=1).astype(np.float32)
+    sd = x.std(axis=1).astype(np.float32) + 1e-12
+    return {
+        "params": random_params(seed),
+        "in_mean": mean, "in_scale": (1.0 / sd).astype(np.float32),
+        "out_mean": np.zeros(14, dtype=np.float32),
+        "out_scale": np.array([1e-7] * 12 + [1e-6, 1e-4], dtype=np.float32),
+        "hull_lo": lo, "hull_hi": hi, "residual_tol": 1e-6,
+    }
+
+
+def test_surrogate_checks_and_fallback(gchem, ochem):
+    n = 600
+    x = random_states(n, 13)
+    sur = toy_surrogate(x)
+    gchem.set_surrogate(sur)
+    ochem.set_surrogate(sur)
+    xf = x.astype(np.float32)
+    ref = np.zeros((14, n), dtype=np.float32)
+    rflags = np.zeros(n, dtype=np.int32)
+    ochem.surrogate(xf, ref, rflags, dt=1e-5, fallback=1)
+    out = dev(np.zeros((14, n), dtype=np.float32))
+    flags = dev(np.zeros(n, dtype=np.int32))
+    gchem.surrogate(dev(xf), out, flags, dt=1e-5, fallback=1)
+    got, gf = host(out), host(flags)
+    assert (rflags == 1).sum() > 50 and (rflags == 0).sum() > 50
+    agree = gf == rflags
+    assert agree.mean() >= 0.98
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+    d = np.abs(got - ref) / scale
+    assert d[:, agree].max() <= 2e-3
+    # fallback cells carry the plain integrator's result
+    plain = dev(np.zeros((14, n)))
+    gchem.integrate(dev(xf.astype(np.float64)), plain, None, dt=1e-5)
+    fb = gf != 0
+    p = host(plain).astype(np.float32)
+    np.testing.assert_allclose(got[:, fb], p[:, fb], rtol=1e-6, atol=1e-30)
+
+
+def coupled_pair(hip, oracle64, precision, shape=(12, 10, 16)):
+    nx, ny, nz = shape
+    A = workloads.pyrolysis12(lib=hip, nx=nx, ny=ny, nz=nz, precision=precision)
+    B = workloads.pyrolysis12(lib=oracle64, nx=nx, ny=ny, nz=nz, precision="fp64")
+    A.init_simulation()
+    B.init_simulation()
+    return A, B
+
+
+def compare_fields(A, B):
+    worst = {}
+    pairs = [("u", A.v, B.v), ("TF", A.TF.S, B.TF.S), ("TS", A.TS.S, B.TS.S), ("rho", A.rho, B.rho)]
+    pairs += [(f"S{j}", a.S, b.S) for j, (a, b) in enumerate(zip(A.species, B.species))]
+    for name, fa, fb in pairs:
+        a, b = fa.to_numpy(), fb.to_numpy()
+        worst[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    return worst
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 2e-5)])
+def test_coupled_step_with_plain_chemistry(hip, oracle64, precision, tol):
+    A, B = coupled_pair(hip, oracle64, precision)
+    ga, gb = chem.Chem(hip), chem.Chem(oracle64)
+    A.attach_chemistry(ga, chem.MODE_PLAIN, rho_solid_ref=A.chem_rho_solid_ref)
+    B.attach_chemistry(gb, chem.MODE_PLAIN, rho_solid_ref=B.chem_rho_solid_ref)
+    A.run(6)
+    B.run(6)
+    w = compare_fields(A, B)
+    print(precision, {k: f"{v:.2e}" for k, v in w.items()})
+    assert max(w.values()) <= tol, w
+    st = A.chemistry_stats(ga)
+    assert st["cells"] == B.chemistry_stats(gb)["cells"] > 0
+    # the operator did something: char appeared
+    assert B.species[13].S.to_numpy().max() > 0
+
+
+def test_coupled_step_with_surrogate(hip, oracle64):
+    A, B = coupled_pair(hip, oracle64, "fp64")
+    ga, gb = chem.Chem(hip), chem.Chem(oracle64)
+    x = random_states(400, 14)
+    sur = toy_surrogate(x)
+    sur["hull_lo"] = (x.min(axis=1) - 1e9).astype(np.float32)
+    sur["hull_hi"] = (x.max(axis=1) + 1e9).astype(np.float32)
+    sur["residual_tol"] = 1e30
+    ga.set_surrogate(sur)
+    gb.set_surrogate(sur)
+    A.attach_chemistry(ga, chem.MODE_SURROGATE, rho_solid_ref=A.chem_rho_solid_ref)
+    B.attach_chemistry(gb, chem.MODE_SURROGATE, rho_solid_ref=B.chem_rho_solid_ref)
+    A.run(4)
+    B.run(4)
+    w = compare_fields(A, B)
+    assert max(w.values()) <= 1e-5, w
+    sa, sb = A.chemistry_stats(ga), B.chemistry_stats(gb)
+    assert sa["cells"] == sb["cells"] and sa["fallback_hull"] == sb["fallback_hull"] == 0

@@ -16,14 +16,16 @@ from conftest import ROOT
 from lbm_porouschem_amd import _abi, constant, flag
 from lbm_porouschem_amd.solver import LBM2DSolver, LBM3DSolver
 
-HEADER = os.path.join(ROOT, "include", "lbm_porouschem.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("lbm_porouschem.h", "lbm_chem.h")]
 
 
-def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(lbm_\w+)\s*\(", text, flags=re.M)
-    assert len(names) >= 15, names
+def declared_functions(headers=HEADERS):
+    names = []
+    for header in headers:
+        text = open(header).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^[A-Za-z_][\w \*]*?\b(lbm_\w+)\s*\(", text, flags=re.M)
+    assert len(names) >= 12, names
     return sorted(set(names))
 
 
@@ -47,7 +49,18 @@ def test_library_exports_every_declared_symbol(which, oracle64):
 
 
 def test_python_binding_names_every_symbol():
-    assert set(_abi._SYMBOLS) == set(declared_functions())
+    from lbm_porouschem_amd import chem
+    assert set(_abi._SYMBOLS) == set(declared_functions(HEADERS[:1]))
+    assert set(chem._CHEM_SYMBOLS) == set(declared_functions(HEADERS[1:]))
+
+
+@pytest.mark.parametrize("which", sorted(libraries()))
+def test_chemistry_struct_layout(which, oracle64):
+    from lbm_porouschem_amd import chem
+    lib = ctypes.CDLL(libraries()[which])
+    lib.lbm_struct_size.restype = ctypes.c_size_t
+    assert lib.lbm_struct_size(2) == ctypes.sizeof(chem.ChemMech)
+    assert lib.lbm_struct_size(3) == ctypes.sizeof(chem.Surrogate)
 
 
 @pytest.mark.parametrize("which", sorted(libraries()))
@@ -59,7 +72,7 @@ def test_struct_layout_and_version(which, oracle64):
 
 
 def test_header_enums_match_python_mirror():
-    text = open(HEADER).read()
+    text = open(HEADERS[0]).read()
     vals = dict((k, int(v)) for k, v in re.findall(r"#define\s+(LBM_\w+)\s+\(?(-?\d+)\)?", text))
     assert vals["LBM_ABI_VERSION"] == _abi.ABI_VERSION
     assert vals["LBM_MAX_SPECIES"] == _abi.MAX_SPECIES
