@@ -88,6 +88,9 @@ typedef struct lbm_chem_mech {
     /* stiff integrator (two-stage Rosenbrock ROS2, Verwer et al. 1999, adaptive) */
     double rtol, atol_c, atol_T, atol_B;   /* tolerances: relative; mol/m^3; K; kg/m^3 */
     int max_steps;                         /* integration fails beyond this many attempted steps */
+    int jac_max_age;                       /* reuse the Jacobian for up to this many accepted steps (it is
+                                              re-evaluated after a rejected step; <= 1: every step).
+                                              ROS2 keeps its order with an inexact Jacobian. */
 } lbm_chem_mech;
 
 /* Surrogate: x_n = (x - in_mean) * in_scale per input, y = y_n * out_scale + out_mean per

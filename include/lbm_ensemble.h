@@ -1,0 +1,59 @@
+/*
+ * lbm_ensemble.h -- ensembles of lattice handles (sensitivity studies over seeds, porosity,
+ * inlet temperature, kinetics) stepped together as captured HIP graphs.
+ *
+ * The reference has no counterpart: a sensitivity study there is a loop of separate runs of
+ * src/LBM/app.py (its log_sensitivity file records such a session).  Here the members are
+ * ordinary handles of lbm_porouschem.h, created, parameterised and initialised one by one;
+ * the ensemble only schedules their time steps:
+ *   - members are bin-packed by cell count into "lanes" (concurrent branches of the graph);
+ *   - one graph holds a whole number of step pairs of every member (the A-A layout returns
+ *     to its starting parity after two steps, so a pair is the unit that can be replayed);
+ *   - lbm_ensemble_tune times candidate lane counts and pairs-per-graph on the device and
+ *     keeps the fastest schedule.
+ * Results are those of stepping every member with lbm_step: the same kernels in the same
+ * order per member.  Members must not carry the chemistry operator, and while they belong
+ * to an ensemble they must be stepped only through it.  Exported by liblbm_mi355x.so.
+ */
+#ifndef LBM_ENSEMBLE_H
+#define LBM_ENSEMBLE_H
+
+#include "lbm_porouschem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ENSEMBLE_MAX_LANES 16
+
+typedef struct lbm_ensemble lbm_ensemble;
+
+/* Group n initialised handles on one device (all at the same step parity).  The handles stay
+ * owned by the caller and must outlive the ensemble. */
+int lbm_ensemble_create(lbm_handle* const* members, int n, lbm_ensemble** out);
+void lbm_ensemble_destroy(lbm_ensemble* e);
+
+/* Use this schedule: lanes in 1..LBM_ENSEMBLE_MAX_LANES, pairs_per_graph >= 1. */
+int lbm_ensemble_set_schedule(lbm_ensemble* e, int lanes, int pairs_per_graph);
+
+/* Autotune the schedule: time every candidate (lanes in {1, 2, 4, 8, 16} up to the member
+ * count, pairs_per_graph in {1, 4}) over trial_pairs step pairs each and keep the fastest.
+ * The trial steps advance the members.  Outputs (optional): the chosen lanes and pairs per
+ * graph, and ms[10] = device time per step pair of each candidate in that order (-1 where
+ * a candidate was skipped). */
+int lbm_ensemble_tune(lbm_ensemble* e, int trial_pairs, int* lanes, int* pairs_per_graph, double* ms);
+
+/* Advance every member nsteps time steps (graph replays; an odd last step runs eagerly).
+ * Returns after the device work has completed. */
+int lbm_ensemble_step(lbm_ensemble* e, int nsteps);
+
+/* Summed device time of lbm_ensemble_step calls (ms) and graph launches since creation. */
+int lbm_ensemble_stats(const lbm_ensemble* e, double* ms, long long* graph_launches);
+
+const char* lbm_ensemble_last_error(const lbm_ensemble* e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_ENSEMBLE_H */

@@ -265,5 +265,12 @@ def load_library():
             raise ImportError(
                 f"{path} is missing: build the gfx950 library first "
                 "(python -c 'import __graft_entry__ as g; g.build()' or python -m lbm_porouschem_amd.build)")
+        # torch (device memory for the chemistry entry points) must bring its HIP runtime into the
+        # process before this library does: the library then binds to that same runtime instead of
+        # a second copy that torch would find without devices
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _PRODUCT = CLib(path)
     return _PRODUCT

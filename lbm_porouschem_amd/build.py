@@ -34,6 +34,7 @@ UNITS = [
     ("lbm_f16.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     # chemistry operator: FP64 integrator rounded like the CPU checker; MLP kernels
     ("chem.hip", ["-ffp-contract=off"]),
+    ("ensemble.hip", []),
 ]
 
 

@@ -41,7 +41,7 @@ class ChemMech(ctypes.Structure):
         ("pyro_A", _d * NPYRO), ("pyro_Ea_R", _d * NPYRO), ("pyro_dH", _d * NPYRO),
         ("pyro_gas", (_d * NSP) * NPYRO), ("biomass_molemass", _d), ("biomass_elements", _d * 4),
         ("char_molemass", _d), ("cs_solid", _d),
-        ("rtol", _d), ("atol_c", _d), ("atol_T", _d), ("atol_B", _d), ("max_steps", _i),
+        ("rtol", _d), ("atol_c", _d), ("atol_T", _d), ("atol_B", _d), ("max_steps", _i), ("jac_max_age", _i),
     ]
 
 
@@ -79,7 +79,7 @@ CHAR_M = 12.011e-3
 
 
 def syngas_mechanism(routes=PYRO_ROUTES, rtol=1e-6, atol_c=1e-9, atol_T=1e-6, atol_B=1e-9, max_steps=20000,
-                     cs_solid=1500.0, table=None):
+                     cs_solid=1500.0, table=None, jac_max_age=10):
     """ChemMech of the GRI-Mech 3.0 syngas sub-mechanism plus the pyrolysis routes."""
     t = table or load_mechanism_table()
     m = ChemMech()
@@ -129,6 +129,7 @@ def syngas_mechanism(routes=PYRO_ROUTES, rtol=1e-6, atol_c=1e-9, atol_T=1e-6, at
     m.char_molemass = CHAR_M
     m.cs_solid = cs_solid
     m.rtol, m.atol_c, m.atol_T, m.atol_B, m.max_steps = rtol, atol_c, atol_T, atol_B, max_steps
+    m.jac_max_age = jac_max_age
     return m
 
 

@@ -53,11 +53,13 @@ def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000,
         x = cell_inputs_from_solver(lb, rho_solid_ref=lb.chem_rho_solid_ref)
         xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
         yd = torch.zeros((NOUT, x.shape[1]), dtype=torch.float64, device="cuda")
-        g.integrate(xd, yd, None, dt=lb.dt)
+        nd = torch.zeros(x.shape[1], dtype=torch.int32, device="cuda")
+        g.integrate(xd, yd, nd, dt=lb.dt)
         xs.append(x)
         ys.append(yd.cpu().numpy())
-        log(f"collect: step {s}: {x.shape[1]} cells, T {x[0].min():.0f}-{x[0].max():.0f} K "
-            f"({time.perf_counter() - t0:.1f} s)")
+        ns = nd.cpu().numpy()
+        log(f"collect: step {s}: {x.shape[1]} cells, T {x[0].min():.0f}-{x[0].max():.0f} K, integrator steps "
+            f"mean {ns.mean():.1f} max {ns.max()} ({time.perf_counter() - t0:.1f} s)")
     X = np.concatenate(xs, axis=1)
     Y = np.concatenate(ys, axis=1)
     if X.shape[1] > max_samples:

@@ -7,14 +7,15 @@
 // operation like the CPU checker.
 //
 // Kernels
-//   k_plain<Src>      one thread per cell: gather inputs, two-stage Rosenbrock (ROS2)
-//                     integration over dt in FP64 (finite-difference Jacobian, pivoted
-//                     LU), hand the result to the source policy.
+//   k_plain<Src>      persistent lanes over a work queue of cells: gather inputs, two-stage
+//                     Rosenbrock (ROS2) integration over dt in FP64 one attempt at a time
+//                     (finite-difference Jacobian reused across accepted steps, pivoted LU),
+//                     hand the result to the source policy, take the next cell.
 //   k_surrogate<Src>  one wavefront per 16 cells: inputs staged through LDS, three MLP
 //                     layers as v_mfma_f32_16x16x16_bf16 tiles with the weights in LDS
 //                     (fragment order, conflict-free 8-byte reads), hull and element-
 //                     balance checks, fallback cells appended to a list.
-//   k_plain_list<Src> the plain integrator over the fallback list.
+//   k_plain<Src> over a list: the plain integrator on the surrogate's fallback cells.
 //   k_train / k_wgrad / k_adam   FP32 training: per-sample forward + backward with
 //                     BF16-rounded operands, per-parameter gradient reduction in FP64,
 //                     Adam update.
@@ -158,140 +159,155 @@ __device__ void rhs(const lbm_chem_mech& m, const TCache& tc, double Csol, const
     f[NS + 1] = -rb;
 }
 
-// Integrate one cell: in[16] -> out[14] over dt.  Returns accepted steps or -1.
-__device__ int integrate_cell(const lbm_chem_mech& m, const double* in, double dt, double* out) {
-    const double T0 = in[0], p = in[1], B0 = in[NS + 2], C0 = in[NS + 3];
+// Integration state of one cell.  The integration is advanced one ROS2 attempt at a time
+// so that a lane whose cell has finished can take the next cell from the work queue while
+// the other lanes of its wavefront continue (cells differ by orders of magnitude in the
+// number of steps they need).
+struct Cell {
     double y[NY], c0[NS];
+    double t, h, Csol;
+    int nacc, natt, rejected, age, have_jac;
+    double Jac[NY][NY];
+};
+
+__device__ void cell_start(const lbm_chem_mech& m, const double* in, double dt, Cell& s) {
+    const double T0 = in[0], p = in[1];
     double sym = 0.0;
 #pragma unroll
     for (int k = 0; k < NS; ++k) sym += in[2 + k] / m.molemass[k];
     const double rho_g = sym > 0.0 ? p / (R_GAS * T0 * sym) : 0.0;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        c0[k] = rho_g * in[2 + k] / m.molemass[k];
-        y[k] = c0[k];
+        s.c0[k] = rho_g * in[2 + k] / m.molemass[k];
+        s.y[k] = s.c0[k];
     }
-    y[NS] = T0;
-    y[NS + 1] = B0;
-#pragma unroll
-    for (int o = 0; o < NO; ++o) out[o] = 0.0;
-    if (!(T0 > 0.0) || !(dt > 0.0)) return -1;
-    const double Csol = (B0 + C0) * m.cs_solid;
-    const double gam = 1.0 + 1.0 / sqrt(2.0);
+    s.y[NS] = T0;
+    s.y[NS + 1] = in[NS + 2];
+    s.Csol = (in[NS + 2] + in[NS + 3]) * m.cs_solid;
+    s.t = 0.0;
+    s.h = dt;
+    s.nacc = s.natt = s.rejected = s.age = s.have_jac = 0;
+}
 
-    double t = 0.0, h = dt;
-    int nacc = 0, natt = 0, rejected = 0, ok = 1;
+// One ROS2 attempt.  Returns 0 while the cell is unfinished, 1 when it reached dt, -1 when it
+// hit max_steps (same step control and Jacobian reuse as the CPU checker).
+__device__ int cell_attempt(const lbm_chem_mech& m, double dt, Cell& s) {
+    const double gam = 1.0 + 1.0 / sqrt(2.0);
+    const int last = s.h >= dt - s.t;
+    if (last) s.h = dt - s.t;
+    if (s.natt >= m.max_steps) return -1;
+    ++s.natt;
+    const double h = s.h;
     TCache tc;
-    double f0[NY], f1[NY], k1[NY], k2[NY], yn[NY], J[NY][NY];
+    double f0[NY], f1[NY], k1[NY], k2[NY], yn[NY], A[NY][NY];
     int piv[NY];
-    for (;;) {
-        const int last = h >= dt - t;
-        if (last) h = dt - t;
-        if (natt >= m.max_steps) {
-            ok = 0;
-            break;
-        }
-        ++natt;
-        tcache_fill(m, y[NS], tc);
-        rhs(m, tc, Csol, y, f0);
+    tcache_fill(m, s.y[NS], tc);
+    rhs(m, tc, s.Csol, s.y, f0);
+    if (!s.have_jac || s.rejected || s.age >= m.jac_max_age) {
         // finite-difference Jacobian, column by column
         for (int j = 0; j < NY; ++j) {
-            const double yj = y[j];
+            const double yj = s.y[j];
             const double floor_j = j < NS ? 1e-6 : (j == NS ? 1.0 : 1e-3);
             const double a = fabs(yj) > floor_j ? fabs(yj) : floor_j;
             const double tmp = yj + 1e-7 * a;
             const double d = tmp - yj;
-            y[j] = tmp;
+            s.y[j] = tmp;
             if (j == NS) {
                 TCache tt;
                 tcache_fill(m, tmp, tt);
-                rhs(m, tt, Csol, y, f1);
+                rhs(m, tt, s.Csol, s.y, f1);
             } else {
-                rhs(m, tc, Csol, y, f1);
+                rhs(m, tc, s.Csol, s.y, f1);
             }
-            y[j] = yj;
-            for (int i = 0; i < NY; ++i) J[i][j] = (f1[i] - f0[i]) / d;
+            s.y[j] = yj;
+            for (int i = 0; i < NY; ++i) s.Jac[i][j] = (f1[i] - f0[i]) / d;
         }
-        for (int i = 0; i < NY; ++i)
-            for (int j = 0; j < NY; ++j) J[i][j] = (i == j ? 1.0 : 0.0) - gam * h * J[i][j];
-        // LU with partial pivoting
-        for (int k = 0; k < NY; ++k) {
-            int pk = k;
-            double best = fabs(J[k][k]);
-            for (int i = k + 1; i < NY; ++i)
-                if (fabs(J[i][k]) > best) {
-                    best = fabs(J[i][k]);
-                    pk = i;
-                }
-            piv[k] = pk;
-            if (pk != k)
-                for (int j = 0; j < NY; ++j) {
-                    const double tt = J[k][j];
-                    J[k][j] = J[pk][j];
-                    J[pk][j] = tt;
-                }
-            if (J[k][k] == 0.0) J[k][k] = 1e-300;
-            const double ip = 1.0 / J[k][k];
-            for (int i = k + 1; i < NY; ++i) {
-                const double l = J[i][k] * ip;
-                J[i][k] = l;
-                for (int j = k + 1; j < NY; ++j) J[i][j] -= l * J[k][j];
-            }
-        }
-        auto solve = [&](double* b) {
-            for (int k = 0; k < NY; ++k)
-                if (piv[k] != k) {
-                    const double tt = b[k];
-                    b[k] = b[piv[k]];
-                    b[piv[k]] = tt;
-                }
-            for (int i = 1; i < NY; ++i)
-                for (int j = 0; j < i; ++j) b[i] -= J[i][j] * b[j];
-            for (int i = NY - 1; i >= 0; --i) {
-                for (int j = i + 1; j < NY; ++j) b[i] -= J[i][j] * b[j];
-                b[i] /= J[i][i];
-            }
-        };
-        for (int i = 0; i < NY; ++i) k1[i] = f0[i];
-        solve(k1);
-        for (int i = 0; i < NY; ++i) yn[i] = y[i] + h * k1[i];
-        tcache_fill(m, yn[NS], tc);
-        rhs(m, tc, Csol, yn, f1);
-        for (int i = 0; i < NY; ++i) k2[i] = f1[i] - 2.0 * k1[i];
-        solve(k2);
-        double err = 0.0;
-        const double atl[3] = {m.atol_c, m.atol_T, m.atol_B};
-        for (int i = 0; i < NY; ++i) {
-            yn[i] = y[i] + 1.5 * h * k1[i] + 0.5 * h * k2[i];
-            const double e = 0.5 * h * (k1[i] + k2[i]);
-            const double ay = fabs(y[i]) > fabs(yn[i]) ? fabs(y[i]) : fabs(yn[i]);
-            const double sc = atl[i < NS ? 0 : i - NS + 1] + m.rtol * ay;
-            err += (e / sc) * (e / sc);
-        }
-        err = sqrt(err / NY);
-        const double es = err > 1e-10 ? err : 1e-10;
-        double fac = 0.9 / sqrt(es);
-        if (err <= 1.0) {
-            for (int i = 0; i < NY; ++i) y[i] = yn[i];
-            ++nacc;
-            if (last) break;
-            t += h;
-            if (fac > 5.0) fac = 5.0;
-            if (fac < 0.2) fac = 0.2;
-            if (rejected && fac > 1.0) fac = 1.0;
-            rejected = 0;
-        } else {
-            if (fac < 0.2) fac = 0.2;
-            if (fac > 1.0) fac = 1.0;
-            rejected = 1;
-        }
-        h *= fac;
+        s.have_jac = 1;
+        s.age = 0;
     }
+    for (int i = 0; i < NY; ++i)
+        for (int j = 0; j < NY; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - gam * h * s.Jac[i][j];
+    // LU with partial pivoting
+    for (int k = 0; k < NY; ++k) {
+        int pk = k;
+        double best = fabs(A[k][k]);
+        for (int i = k + 1; i < NY; ++i)
+            if (fabs(A[i][k]) > best) {
+                best = fabs(A[i][k]);
+                pk = i;
+            }
+        piv[k] = pk;
+        if (pk != k)
+            for (int j = 0; j < NY; ++j) {
+                const double tt = A[k][j];
+                A[k][j] = A[pk][j];
+                A[pk][j] = tt;
+            }
+        if (A[k][k] == 0.0) A[k][k] = 1e-300;
+        const double ip = 1.0 / A[k][k];
+        for (int i = k + 1; i < NY; ++i) {
+            const double l = A[i][k] * ip;
+            A[i][k] = l;
+            for (int j = k + 1; j < NY; ++j) A[i][j] -= l * A[k][j];
+        }
+    }
+    auto solve = [&](double* b) {
+        for (int k = 0; k < NY; ++k)
+            if (piv[k] != k) {
+                const double tt = b[k];
+                b[k] = b[piv[k]];
+                b[piv[k]] = tt;
+            }
+        for (int i = 1; i < NY; ++i)
+            for (int j = 0; j < i; ++j) b[i] -= A[i][j] * b[j];
+        for (int i = NY - 1; i >= 0; --i) {
+            for (int j = i + 1; j < NY; ++j) b[i] -= A[i][j] * b[j];
+            b[i] /= A[i][i];
+        }
+    };
+    for (int i = 0; i < NY; ++i) k1[i] = f0[i];
+    solve(k1);
+    for (int i = 0; i < NY; ++i) yn[i] = s.y[i] + h * k1[i];
+    tcache_fill(m, yn[NS], tc);
+    rhs(m, tc, s.Csol, yn, f1);
+    for (int i = 0; i < NY; ++i) k2[i] = f1[i] - 2.0 * k1[i];
+    solve(k2);
+    double err = 0.0;
+    const double atl[3] = {m.atol_c, m.atol_T, m.atol_B};
+    for (int i = 0; i < NY; ++i) {
+        yn[i] = s.y[i] + 1.5 * h * k1[i] + 0.5 * h * k2[i];
+        const double e = 0.5 * h * (k1[i] + k2[i]);
+        const double ay = fabs(s.y[i]) > fabs(yn[i]) ? fabs(s.y[i]) : fabs(yn[i]);
+        const double sc = atl[i < NS ? 0 : i - NS + 1] + m.rtol * ay;
+        err += (e / sc) * (e / sc);
+    }
+    err = sqrt(err / NY);
+    const double es = err > 1e-10 ? err : 1e-10;
+    double fac = 0.9 / sqrt(es);
+    if (err <= 1.0) {
+        for (int i = 0; i < NY; ++i) s.y[i] = yn[i];
+        ++s.nacc;
+        ++s.age;
+        if (last) return 1;
+        s.t += h;
+        if (fac > 5.0) fac = 5.0;
+        if (fac < 0.2) fac = 0.2;
+        if (s.rejected && fac > 1.0) fac = 1.0;
+        s.rejected = 0;
+    } else {
+        if (fac < 0.2) fac = 0.2;
+        if (fac > 1.0) fac = 1.0;
+        s.rejected = 1;
+    }
+    s.h *= fac;
+    return 0;
+}
+
+__device__ void cell_result(const lbm_chem_mech& m, const double* in, const Cell& s, double* out) {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) out[k] = (y[k] - c0[k]) * m.molemass[k];
-    out[NS] = y[NS + 1] - B0;
-    out[NS + 1] = y[NS] - T0;
-    return ok ? nacc : -1;
+    for (int k = 0; k < NS; ++k) out[k] = (s.y[k] - s.c0[k]) * m.molemass[k];
+    out[NS] = s.y[NS + 1] - in[NS + 2];
+    out[NS + 1] = s.y[NS] - in[0];
 }
 
 // element-balance residual (lbm_chem_residual)
@@ -449,33 +465,57 @@ template <class Real> struct SrcFields {
 // ---------------------------------------------------------------------------
 // plain kernels
 // ---------------------------------------------------------------------------
+// Persistent plain integration over a work queue of n entries (entry j = cell j, or list[j]
+// when a list is given; the entry count is read from `count` when it is non-null).  Each lane
+// holds one cell; a lane whose cell finished (or whose entry is not a live cell) takes the
+// next entry, so a wavefront keeps all of its lanes busy until the queue is empty.  Every
+// lane leaves the loop once the queue is drained and its cell finished.
 template <class Src>
 __global__ __launch_bounds__(PLAIN_BLOCK) void k_plain(const lbm_chem_mech* __restrict__ mech, Src src, long long n,
-                                                       double dt, unsigned long long* stats) {
-    const long long i = (long long)blockIdx.x * PLAIN_BLOCK + threadIdx.x;
-    bool act = false;
-    if (i < n) {
-        double x[NI], y[NO];
-        act = src.load(i, x);
-        if (act) {
-            const int st = integrate_cell(*mech, x, dt, y);
-            src.plain(i, x, y, st);
+                                                       const unsigned* __restrict__ list,
+                                                       const unsigned* __restrict__ count, double dt,
+                                                       unsigned* __restrict__ next,
+                                                       unsigned long long* __restrict__ stats) {
+    const lbm_chem_mech& m = *mech;
+    const long long total = count ? (long long)*count : n;
+    Cell s;
+    double x[NI];
+    long long cell = -1;
+    bool drained = false;
+    for (;;) {
+        const bool want = cell < 0 && !drained;
+        const unsigned long long bits = __ballot(want);
+        bool got = false;
+        if (bits) {
+            const int leader = __ffsll((long long)bits) - 1;
+            unsigned base = 0;
+            if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(next, (unsigned)__popcll(bits));
+            base = __shfl(base, leader, 64);
+            if (want) {
+                const long long j = (long long)base + lane_prefix(bits);
+                if (j < total) {
+                    const long long i = list ? (long long)list[j] : j;
+                    if (src.load(i, x)) {
+                        cell_start(m, x, dt, s);
+                        cell = i;
+                        got = true;
+                    }
+                } else {
+                    drained = true;
+                }
+            }
         }
-    }
-    count3(act, false, false, stats);
-}
-
-template <class Src>
-__global__ __launch_bounds__(PLAIN_BLOCK) void k_plain_list(const lbm_chem_mech* __restrict__ mech, Src src,
-                                                            const unsigned* __restrict__ list,
-                                                            const unsigned* __restrict__ count, double dt) {
-    const unsigned cnt = *count;
-    for (unsigned j = blockIdx.x * PLAIN_BLOCK + threadIdx.x; j < cnt; j += gridDim.x * PLAIN_BLOCK) {
-        const long long i = list[j];
-        double x[NI], y[NO];
-        src.load(i, x);
-        const int st = integrate_cell(*mech, x, dt, y);
-        src.plain(i, x, y, st);
+        count3(got, false, false, stats);
+        if (__ballot(cell >= 0 || !drained) == 0) break;
+        if (cell >= 0) {
+            const int st = cell_attempt(m, dt, s);
+            if (st != 0) {
+                double y[NO];
+                cell_result(m, x, s, y);
+                src.plain(cell, x, y, st > 0 ? s.nacc : -1);
+                cell = -1;
+            }
+        }
     }
 }
 
@@ -847,7 +887,12 @@ unsigned grid_tiles(const lbm_chem* c, long long n) {
     return (unsigned)(tiles < cap ? (tiles > 0 ? tiles : 1) : cap);
 }
 
-unsigned grid_plain(long long n) { return (unsigned)((n + PLAIN_BLOCK - 1) / PLAIN_BLOCK); }
+// persistent plain pass: enough wavefronts to fill the device, never more than the work
+unsigned grid_plain(const lbm_chem* c, long long n) {
+    const long long blocks = (n + PLAIN_BLOCK - 1) / PLAIN_BLOCK;
+    const long long cap = (long long)c->n_cu * 8;
+    return (unsigned)(blocks < cap ? (blocks > 0 ? blocks : 1) : cap);
+}
 
 }  // namespace
 
@@ -859,16 +904,20 @@ int chem_device(const lbm_chem* c) { return c ? c->device : -1; }
 template <class Real> int chem_run_fields(lbm_chem* c, int mode, const ChemFields<Real>& f, hipStream_t stream) {
     SrcFields<Real> src{f, c->d_molemass};
     if (f.N <= 0) return LBM_OK;
+    // f.count[0]: fallback-list length, f.count[1]: work-queue head of the plain pass
     if (mode == LBM_CHEM_PLAIN) {
-        hipLaunchKernelGGL((k_plain<SrcFields<Real>>), dim3(grid_plain(f.N)), dim3(PLAIN_BLOCK), 0, stream, c->d_mech,
-                           src, f.N, f.dt, f.stats);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, f.count, 2);
+        hipLaunchKernelGGL((k_plain<SrcFields<Real>>), dim3(grid_plain(c, f.N)), dim3(PLAIN_BLOCK), 0, stream,
+                           c->d_mech, src, f.N, (const unsigned*)nullptr, (const unsigned*)nullptr, f.dt, f.count + 1,
+                           f.stats);
     } else {
         if (!c->have_sur) return cerr(c, LBM_ERR_STATE, "no surrogate loaded");
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, f.count, 1);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, f.count, 2);
         hipLaunchKernelGGL((k_surrogate<SrcFields<Real>, 1>), dim3(grid_tiles(c, f.N)), dim3(SUR_BLOCK), 0, stream,
                            c->d_sur, c->d_mech, src, f.N, f.list, f.count, f.stats);
-        hipLaunchKernelGGL((k_plain_list<SrcFields<Real>>), dim3((unsigned)c->n_cu * 4), dim3(PLAIN_BLOCK), 0, stream,
-                           c->d_mech, src, f.list, f.count, f.dt);
+        hipLaunchKernelGGL((k_plain<SrcFields<Real>>), dim3(grid_plain(c, f.N)), dim3(PLAIN_BLOCK), 0, stream,
+                           c->d_mech, src, f.N, (const unsigned*)f.list, (const unsigned*)f.count, f.dt, f.count + 1,
+                           (unsigned long long*)nullptr);
     }
     CHEM_TRY(hipGetLastError());
     return LBM_OK;
@@ -907,7 +956,7 @@ int lbm_chem_create(const lbm_chem_mech* mech, int device, lbm_chem** out) {
         if (hipMalloc(&c->d_sur, sizeof(SurDev)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMalloc(&c->d_tmp, sizeof(SurDev)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMalloc(&c->d_params, NPARAM * sizeof(float)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
-        if (hipMalloc(&c->d_count, sizeof(unsigned)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
+        if (hipMalloc(&c->d_count, 2 * sizeof(unsigned)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMemcpy(c->d_mech, mech, sizeof(lbm_chem_mech), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
         if (hipMemcpy(c->d_molemass, mech->molemass, NS * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
     } while (0);
@@ -939,7 +988,9 @@ int lbm_chem_integrate(lbm_chem* c, const double* in, double* out, int* nsteps, 
     if (n == 0) return LBM_OK;
     CHEM_TRY(hipSetDevice(c->device));
     SrcF64 src{in, out, nsteps, n};
-    hipLaunchKernelGGL((k_plain<SrcF64>), dim3(grid_plain(n)), dim3(PLAIN_BLOCK), 0, c->stream, c->d_mech, src, n, dt,
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->d_count, 2);
+    hipLaunchKernelGGL((k_plain<SrcF64>), dim3(grid_plain(c, n)), dim3(PLAIN_BLOCK), 0, c->stream, c->d_mech, src, n,
+                       (const unsigned*)nullptr, (const unsigned*)nullptr, dt, c->d_count + 1,
                        (unsigned long long*)nullptr);
     CHEM_TRY(hipGetLastError());
     CHEM_TRY(hipStreamSynchronize(c->stream));
@@ -986,13 +1037,14 @@ int lbm_chem_surrogate(lbm_chem* c, const float* in, float* out, int* flags, lon
     int rc = ensure_list(c, n);
     if (rc) return rc;
     SrcF32 src{in, out, flags, n, fallback, c->d_list, c->d_count};
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->d_count, 1);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->d_count, 2);
     hipLaunchKernelGGL((k_surrogate<SrcF32, 1>), dim3(grid_tiles(c, n)), dim3(SUR_BLOCK), 0, c->stream, c->d_sur,
                        c->d_mech, src, n, fallback ? c->d_list : (unsigned*)nullptr, c->d_count,
                        (unsigned long long*)nullptr);
     if (fallback)
-        hipLaunchKernelGGL((k_plain_list<SrcF32>), dim3((unsigned)c->n_cu * 4), dim3(PLAIN_BLOCK), 0, c->stream,
-                           c->d_mech, src, c->d_list, c->d_count, dt);
+        hipLaunchKernelGGL((k_plain<SrcF32>), dim3(grid_plain(c, n)), dim3(PLAIN_BLOCK), 0, c->stream, c->d_mech, src,
+                           n, (const unsigned*)c->d_list, (const unsigned*)c->d_count, dt, c->d_count + 1,
+                           (unsigned long long*)nullptr);
     CHEM_TRY(hipGetLastError());
     CHEM_TRY(hipStreamSynchronize(c->stream));
     return LBM_OK;

@@ -21,7 +21,7 @@ template <class Real> struct ChemFields {
     double p0, rho_ref, dt;
     Real* xsrc;          // [16][N]: 12 species dS, biomass dS, char dS, dTF, dTS (lattice units)
     unsigned* list;      // [N] fallback cells of the surrogate pass
-    unsigned* count;     // [1]
+    unsigned* count;     // [2]: fallback-list length, work-queue head
     unsigned long long* stats;  // [3]: cells, hull fallbacks, residual fallbacks
 };
 

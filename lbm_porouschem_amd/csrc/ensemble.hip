@@ -1,0 +1,305 @@
+// ensemble.hip -- lbm_ensemble.h: members' time steps captured into HIP graphs with one
+// branch ("lane") per group of members, replayed per step pair; schedule autotuning.
+#include "lbm_impl.hpp"
+#include "../../include/lbm_ensemble.h"
+
+#include <algorithm>
+#include <numeric>
+
+using namespace lbmimpl;
+
+struct lbm_ensemble {
+    std::vector<lbm_handle*> m;
+    std::vector<int> lane_of;
+    int device = 0;
+    hipStream_t s0 = nullptr;
+    hipStream_t lane[LBM_ENSEMBLE_MAX_LANES] = {};
+    hipEvent_t fork = nullptr, join[LBM_ENSEMBLE_MAX_LANES] = {}, t0 = nullptr, t1 = nullptr;
+    int nl = 1, ppg = 1;
+    // graphs of the current schedule: mid = ppg pairs, one = 1 pair, last = 1 pair whose final
+    // step refreshes every cell's macroscopic fields
+    hipGraphExec_t g_mid = nullptr, g_one = nullptr, g_last = nullptr;
+    int cached_nl = 0, cached_ppg = 0;
+    long long cached_parity = -1, cached_gen = -1;
+    double ms = 0;
+    long long launches = 0;
+    char err[512] = {0};
+};
+
+namespace {
+
+int eerr(lbm_ensemble* e, int code, const char* fmt, ...) {
+    if (e) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(e->err, sizeof(e->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define ENS_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return eerr(e, LBM_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                         \
+    } while (0)
+
+void drop_graphs(lbm_ensemble* e) {
+    for (hipGraphExec_t* g : {&e->g_mid, &e->g_one, &e->g_last})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+    e->cached_nl = e->cached_ppg = 0;
+    e->cached_parity = e->cached_gen = -1;
+}
+
+long long generation(const lbm_ensemble* e) {
+    long long g = 0;
+    for (auto* h : e->m) g += h->gen;
+    return g;
+}
+
+// largest member first into the least-loaded lane
+void assign_lanes(lbm_ensemble* e) {
+    const int n = (int)e->m.size();
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return e->m[a]->impl->cells() > e->m[b]->impl->cells(); });
+    std::vector<long long> load(e->nl, 0);
+    e->lane_of.assign(n, 0);
+    for (int i : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        e->lane_of[i] = l;
+        load[l] += e->m[i]->impl->cells();
+    }
+}
+
+int capture(lbm_ensemble* e, int pairs, bool refresh_last, hipGraphExec_t* out) {
+    *out = nullptr;
+    ENS_TRY(hipStreamBeginCapture(e->s0, hipStreamCaptureModeRelaxed));
+    int rc = LBM_OK;
+    hipError_t he = hipEventRecord(e->fork, e->s0);
+    for (int l = 0; l < e->nl && he == hipSuccess; ++l) he = hipStreamWaitEvent(e->lane[l], e->fork, 0);
+    for (int p = 0; p < pairs && he == hipSuccess && rc == LBM_OK; ++p)
+        for (int k = 0; k < 2 && rc == LBM_OK; ++k)
+            for (size_t i = 0; i < e->m.size() && rc == LBM_OK; ++i) {
+                lbm_handle* h = e->m[i];
+                const long long stepno = h->steps + 1 + 2 * p + k;
+                const int wm = refresh_last && p == pairs - 1 && k == 1;
+                rc = h->impl->enqueue(e->lane[e->lane_of[i]], stepno, wm);
+                if (rc) eerr(e, rc, "member %zu: %s", i, h->err);
+            }
+    for (int l = 0; l < e->nl && he == hipSuccess; ++l) {
+        he = hipEventRecord(e->join[l], e->lane[l]);
+        if (he == hipSuccess) he = hipStreamWaitEvent(e->s0, e->join[l], 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t he2 = hipStreamEndCapture(e->s0, &g);
+    if (rc == LBM_OK && he != hipSuccess) rc = eerr(e, LBM_ERR_DEVICE, "graph capture: %s", hipGetErrorString(he));
+    if (rc == LBM_OK && he2 != hipSuccess) rc = eerr(e, LBM_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(he2));
+    if (rc == LBM_OK) {
+        he = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        if (he != hipSuccess) rc = eerr(e, LBM_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(he));
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+int prepare(lbm_ensemble* e) {
+    const long long parity = e->m[0]->steps & 1;
+    for (auto* h : e->m)
+        if ((h->steps & 1) != parity) return eerr(e, LBM_ERR_STATE, "members are at different step parities");
+    const long long gen = generation(e);
+    if (e->g_mid && e->cached_nl == e->nl && e->cached_ppg == e->ppg && e->cached_parity == parity &&
+        e->cached_gen == gen)
+        return LBM_OK;
+    drop_graphs(e);
+    assign_lanes(e);
+    int rc;
+    if ((rc = capture(e, e->ppg, false, &e->g_mid))) return rc;
+    if ((rc = capture(e, 1, false, &e->g_one))) return rc;
+    if ((rc = capture(e, 1, true, &e->g_last))) return rc;
+    e->cached_nl = e->nl;
+    e->cached_ppg = e->ppg;
+    e->cached_parity = parity;
+    e->cached_gen = gen;
+    return LBM_OK;
+}
+
+void advance(lbm_ensemble* e, int steps) {
+    for (auto* h : e->m) h->steps += steps;
+}
+
+// queue nsteps on s0 (no synchronisation)
+int queue_steps(lbm_ensemble* e, int nsteps) {
+    int rc = prepare(e);
+    if (rc) return rc;
+    const int pairs = nsteps / 2;
+    const bool odd = nsteps & 1;
+    int body = odd ? pairs : pairs - 1;
+    while (body >= e->ppg) {
+        ENS_TRY(hipGraphLaunch(e->g_mid, e->s0));
+        e->launches++;
+        advance(e, 2 * e->ppg);
+        body -= e->ppg;
+    }
+    while (body > 0) {
+        ENS_TRY(hipGraphLaunch(e->g_one, e->s0));
+        e->launches++;
+        advance(e, 2);
+        --body;
+    }
+    if (pairs > 0 && !odd) {
+        ENS_TRY(hipGraphLaunch(e->g_last, e->s0));
+        e->launches++;
+        advance(e, 2);
+    }
+    if (odd) {
+        for (auto* h : e->m) {
+            rc = h->impl->enqueue(e->s0, h->steps + 1, 1);
+            if (rc) return eerr(e, rc, "%s", h->err);
+        }
+        advance(e, 1);
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_ensemble_create(lbm_handle* const* members, int n, lbm_ensemble** out) {
+    if (!members || !out || n < 1) return LBM_ERR_INVALID;
+    *out = nullptr;
+    const int dev = members[0] ? members[0]->d.device : -1;
+    for (int i = 0; i < n; ++i) {
+        if (!members[i] || !members[i]->initialised) return LBM_ERR_STATE;
+        if (members[i]->d.device != dev) return LBM_ERR_INVALID;
+        for (int j = 0; j < i; ++j)
+            if (members[j] == members[i]) return LBM_ERR_INVALID;
+    }
+    lbm_ensemble* e = new (std::nothrow) lbm_ensemble();
+    if (!e) return LBM_ERR_OOM;
+    e->m.assign(members, members + n);
+    e->device = dev;
+    int rc = LBM_OK;
+    do {
+        if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&e->s0, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreate(&e->t0) != hipSuccess || hipEventCreate(&e->t1) != hipSuccess) {
+            rc = LBM_ERR_DEVICE;
+            break;
+        }
+        for (int l = 0; l < LBM_ENSEMBLE_MAX_LANES && rc == LBM_OK; ++l)
+            if (hipStreamCreateWithFlags(&e->lane[l], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&e->join[l], hipEventDisableTiming) != hipSuccess)
+                rc = LBM_ERR_DEVICE;
+    } while (0);
+    if (rc) {
+        lbm_ensemble_destroy(e);
+        return rc;
+    }
+    e->nl = std::min(n, 4);
+    *out = e;
+    return LBM_OK;
+}
+
+void lbm_ensemble_destroy(lbm_ensemble* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->s0) (void)hipStreamSynchronize(e->s0);
+    drop_graphs(e);
+    for (int l = 0; l < LBM_ENSEMBLE_MAX_LANES; ++l) {
+        if (e->lane[l]) (void)hipStreamDestroy(e->lane[l]);
+        if (e->join[l]) (void)hipEventDestroy(e->join[l]);
+    }
+    for (hipEvent_t ev : {e->fork, e->t0, e->t1})
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->s0) (void)hipStreamDestroy(e->s0);
+    delete e;
+}
+
+int lbm_ensemble_set_schedule(lbm_ensemble* e, int lanes, int pairs_per_graph) {
+    if (!e || lanes < 1 || lanes > LBM_ENSEMBLE_MAX_LANES || pairs_per_graph < 1) return LBM_ERR_INVALID;
+    e->nl = lanes;
+    e->ppg = pairs_per_graph;
+    return LBM_OK;
+}
+
+int lbm_ensemble_step(lbm_ensemble* e, int nsteps) {
+    if (!e || nsteps < 0) return LBM_ERR_INVALID;
+    if (nsteps == 0) return LBM_OK;
+    ENS_TRY(hipSetDevice(e->device));
+    for (auto* h : e->m) ENS_TRY(hipStreamSynchronize(h->impl->own_stream()));
+    ENS_TRY(hipEventRecord(e->t0, e->s0));
+    int rc = queue_steps(e, nsteps);
+    if (rc) {
+        (void)hipStreamSynchronize(e->s0);
+        return rc;
+    }
+    ENS_TRY(hipEventRecord(e->t1, e->s0));
+    ENS_TRY(hipEventSynchronize(e->t1));
+    float t = 0;
+    ENS_TRY(hipEventElapsedTime(&t, e->t0, e->t1));
+    e->ms += t;
+    return LBM_OK;
+}
+
+int lbm_ensemble_tune(lbm_ensemble* e, int trial_pairs, int* lanes, int* pairs_per_graph, double* ms) {
+    if (!e || trial_pairs < 1) return LBM_ERR_INVALID;
+    ENS_TRY(hipSetDevice(e->device));
+    for (auto* h : e->m) ENS_TRY(hipStreamSynchronize(h->impl->own_stream()));
+    const int cand_l[5] = {1, 2, 4, 8, 16}, cand_p[2] = {1, 4};
+    double best = 1e300;
+    int bl = e->nl, bp = e->ppg;
+    for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int slot = a * 2 + b;
+            if (ms) ms[slot] = -1.0;
+            if (cand_l[a] > (int)e->m.size() || cand_p[b] > trial_pairs) continue;
+            e->nl = cand_l[a];
+            e->ppg = cand_p[b];
+            // warm-up: capture the schedule's graphs and run one body's worth of pairs (an even
+            // step count keeps the parity the graphs were captured for)
+            int rc = queue_steps(e, 2 * e->ppg);
+            if (rc) return rc;
+            const int reps = std::max(1, trial_pairs / e->ppg);
+            ENS_TRY(hipEventRecord(e->t0, e->s0));
+            for (int r = 0; r < reps; ++r) {
+                ENS_TRY(hipGraphLaunch(e->g_mid, e->s0));
+                e->launches++;
+                advance(e, 2 * e->ppg);
+            }
+            ENS_TRY(hipEventRecord(e->t1, e->s0));
+            ENS_TRY(hipEventSynchronize(e->t1));
+            float t = 0;
+            ENS_TRY(hipEventElapsedTime(&t, e->t0, e->t1));
+            const double per_pair = t / ((double)reps * e->ppg);
+            if (ms) ms[slot] = per_pair;
+            if (per_pair < best) {
+                best = per_pair;
+                bl = e->nl;
+                bp = e->ppg;
+            }
+        }
+    e->nl = bl;
+    e->ppg = bp;
+    if (lanes) *lanes = bl;
+    if (pairs_per_graph) *pairs_per_graph = bp;
+    return LBM_OK;
+}
+
+int lbm_ensemble_stats(const lbm_ensemble* e, double* ms, long long* graph_launches) {
+    if (!e) return LBM_ERR_INVALID;
+    if (ms) *ms = e->ms;
+    if (graph_launches) *graph_launches = e->launches;
+    return LBM_OK;
+}
+
+const char* lbm_ensemble_last_error(const lbm_ensemble* e) { return e ? e->err : "null ensemble"; }
+
+}  // extern "C"

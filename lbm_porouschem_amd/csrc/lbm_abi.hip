@@ -81,6 +81,7 @@ int lbm_set_params(lbm_handle* h, const lbm_params* p) {
     if (h->d.dim == 2 && p->bc_model != LBM_BCMODEL_NEE && p->bc_model != LBM_BCMODEL_NONE)
         return set_err(h, LBM_ERR_UNSUPPORTED, "only the NEE boundary model is implemented");
     h->p = *p;
+    h->gen++;
     int rc = h->impl->set_params(*p);
     if (rc == LBM_OK) h->params_set = 1;
     return rc;
@@ -102,6 +103,7 @@ int lbm_init(lbm_handle* h) {
     if (!h) return LBM_ERR_INVALID;
     if (!h->params_set) return set_err(h, LBM_ERR_STATE, "lbm_set_params must precede lbm_init");
     h->initialised = 0;
+    h->gen++;
     int rc = h->impl->init();
     if (rc) return rc;
     h->initialised = 1;
@@ -150,6 +152,7 @@ int lbm_get_state(lbm_handle* h, void* host, size_t bytes) {
 
 int lbm_set_state(lbm_handle* h, const void* host, size_t bytes) {
     if (!h || !host) return LBM_ERR_INVALID;
+    h->gen++;
     return h->impl->set_state(host, bytes);
 }
 
@@ -168,6 +171,7 @@ const char* lbm_last_error(const lbm_handle* h) { return h ? h->err : "null hand
 int lbm_chem_attach(lbm_handle* h, lbm_chem* c, int mode, double p0, double rho_solid_ref) {
     if (!h) return LBM_ERR_INVALID;
     if (!h->initialised && mode != LBM_CHEM_OFF) return set_err(h, LBM_ERR_STATE, "lbm_init must precede lbm_chem_attach");
+    h->gen++;
     return h->impl->chem_attach(c, mode, p0, rho_solid_ref);
 }
 

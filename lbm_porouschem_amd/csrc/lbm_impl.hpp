@@ -238,6 +238,9 @@ struct Impl {
     virtual int profile(int enable) = 0;
     virtual int profile_read(int kernel, double* ms, long long* n) = 0;
     virtual int chem_attach(lbm_chem* c, int mode, double p0, double rho_ref) = 0;
+    virtual int enqueue(hipStream_t st, long long stepno, int wm) = 0;
+    virtual long long cells() const = 0;
+    virtual hipStream_t own_stream() const = 0;
     virtual int chem_stats(long long* cells, long long* hull, long long* res, double* ms) = 0;
 };
 
@@ -248,6 +251,7 @@ struct lbm_handle {
     lbm_params p;
     int params_set = 0, initialised = 0;
     long long steps = 0;
+    long long gen = 0;  // bumped whenever buffers or launch sequence may change (ensemble graph cache)
     char err[512];
     std::unique_ptr<lbmimpl::Impl> impl;
 };
@@ -816,49 +820,65 @@ struct ImplT final : Impl {
     }
 
     int step(int n) override {
-        const lbm_params& p = h_->p;
         for (int s = 0; s < n; ++s) {
             const long long stepno = h_->steps + 1;  // step 1 reads the layout written by the init collision
             // the chemistry operator reads every cell's fields after each step
             const int wm = s == n - 1 || chem_mode != 0;
-            hipEvent_t e_step = ev_begin(3);
-            if (D == 2 && p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW)
-                hipLaunchKernelGGL((k_inflow2d<Real, Store>), dim3(1), dim3(BLOCK), 0, stream, dP, B, (Real)p.flow_bc[0]);
-            hipEvent_t e_bulk = ev_begin(0);
-            launch_bulk((stepno & 1) != 0, wm);
-            ev_end(0, e_bulk);
-            hipEvent_t e_nee = ev_begin(1);
-            if (p.bc_model == LBM_BCMODEL_NEE) {
-                const int nparts = 1 + (h_->d.thermal ? 2 : 0) + (h_->d.chemical ? nf : 0);
-                for (int fam = 0; fam < D; ++fam) {
-                    const bool fastfam = fam == D - 1;
-                    const long long ncell = fastfam ? nfrows : (fam == 0 ? (long long)ny * nz : (long long)nx * nz);
-                    if (ncell == 0) continue;
-                    hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for(ncell * 2 * nparts)), dim3(BLOCK), 0,
-                                       stream, dP, B, fam, nparts, (const uint32_t*)frows, nfrows);
-                }
-            }
-            ev_end(1, e_nee);
-            hipEvent_t e_face = ev_begin(2);
-            if (nflist) {
-                if (stepno & 1)
-                    hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1>), dim3(grid_for(nflist)), dim3(BLOCK), 0, stream,
-                                       dP, B);
-                else
-                    hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0>), dim3(grid_for(nflist)), dim3(BLOCK), 0, stream,
-                                       dP, B);
-            }
-            ev_end(2, e_face);
-            if (chem_mode) {
-                int rc = run_chem();
-                if (rc) return rc;
-            }
-            ev_end(3, e_step);
-            HIP_TRY(hipGetLastError());
+            int rc = enqueue_step(stream, stepno, wm, true);
+            if (rc) return rc;
             h_->steps++;
         }
         return LBM_OK;
     }
+
+    // Queue the kernels of one time step on `st`.  With eager = false (graph capture by an
+    // ensemble) no profiling events are recorded and the chemistry operator is not run.
+    int enqueue_step(hipStream_t st, long long stepno, int wm, bool eager) {
+        const lbm_params& p = h_->p;
+        hipEvent_t e_step = eager ? ev_begin(3) : nullptr;
+        if (D == 2 && p.bc_flow[0] == LBM_BCFLOW_INLET_FLOW)
+            hipLaunchKernelGGL((k_inflow2d<Real, Store>), dim3(1), dim3(BLOCK), 0, st, dP, B, (Real)p.flow_bc[0]);
+        hipEvent_t e_bulk = eager ? ev_begin(0) : nullptr;
+        launch_bulk(st, (stepno & 1) != 0, wm);
+        ev_end(0, e_bulk);
+        hipEvent_t e_nee = eager ? ev_begin(1) : nullptr;
+        if (p.bc_model == LBM_BCMODEL_NEE) {
+            const int nparts = 1 + (h_->d.thermal ? 2 : 0) + (h_->d.chemical ? nf : 0);
+            for (int fam = 0; fam < D; ++fam) {
+                const bool fastfam = fam == D - 1;
+                const long long ncell = fastfam ? nfrows : (fam == 0 ? (long long)ny * nz : (long long)nx * nz);
+                if (ncell == 0) continue;
+                hipLaunchKernelGGL((k_nee<Real, Store, D, NF, NX>), dim3(grid_for(ncell * 2 * nparts)), dim3(BLOCK), 0,
+                                   st, dP, B, fam, nparts, (const uint32_t*)frows, nfrows);
+            }
+        }
+        ev_end(1, e_nee);
+        hipEvent_t e_face = eager ? ev_begin(2) : nullptr;
+        if (nflist) {
+            if (stepno & 1)
+                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1>), dim3(grid_for(nflist)), dim3(BLOCK), 0, st, dP,
+                                   B);
+            else
+                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0>), dim3(grid_for(nflist)), dim3(BLOCK), 0, st, dP,
+                                   B);
+        }
+        ev_end(2, e_face);
+        if (eager && chem_mode) {
+            int rc = run_chem();
+            if (rc) return rc;
+        }
+        ev_end(3, e_step);
+        HIP_TRY(hipGetLastError());
+        return LBM_OK;
+    }
+
+    // ---- ensemble members (lbm_ensemble.h) ----
+    int enqueue(hipStream_t st, long long stepno, int wm) override {
+        if (chem_mode) return set_err(h_, LBM_ERR_UNSUPPORTED, "ensembles do not carry the chemistry operator");
+        return enqueue_step(st, stepno, wm, false);
+    }
+    long long cells() const override { return N; }
+    hipStream_t own_stream() const override { return stream; }
 
     // ---- chemistry operator ----
     int run_chem() {
@@ -922,7 +942,7 @@ struct ImplT final : Impl {
             if (!xsrc_buf) {
                 if ((rc = dalloc(&xsrc_buf, 16 * (size_t)N))) return rc;
                 if ((rc = dalloc(&chem_list, (size_t)N))) return rc;
-                if ((rc = dalloc(&chem_count, 1))) return rc;
+                if ((rc = dalloc(&chem_count, 2))) return rc;
                 if ((rc = dalloc(&chem_cnt, 3))) return rc;
             }
             // no sources before the first step after attaching
@@ -956,7 +976,7 @@ struct ImplT final : Impl {
         return LBM_OK;
     }
 
-    void launch_bulk(bool odd, int wm) {
+    void launch_bulk(hipStream_t stream, bool odd, int wm) {
         const dim3 grid(grid_for(N)), block(BLOCK);
         if constexpr (D == 3 && NF == 4 && NX == 2) {
             if (coupled) {

@@ -1,0 +1,159 @@
+"""Ensembles of solvers stepped as captured HIP graphs (include/lbm_ensemble.h), and the
+seeded sensitivity ensemble of the coupled pyrolysis case.
+
+    members = sensitivity_members(64, seed=7)       # solvers, uneven 2D and 3D sizes
+    for lb in members: lb.init_simulation()
+    ens = Ensemble(members)
+    ens.tune(trial_pairs=4)                         # pick lanes / pairs per graph on the device
+    ens.run(1000)
+
+The reference runs a sensitivity study as separate runs of its application (one process
+per parameter set); here the members share one device and their steps are replayed as
+graphs whose branches run concurrently.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+
+_ENS_SYMBOLS = ["lbm_ensemble_create", "lbm_ensemble_destroy", "lbm_ensemble_set_schedule", "lbm_ensemble_tune",
+                "lbm_ensemble_step", "lbm_ensemble_stats", "lbm_ensemble_last_error"]
+
+LANE_CANDIDATES = (1, 2, 4, 8, 16)
+PAIR_CANDIDATES = (1, 4)
+
+
+def _bind(lib):
+    lib.lbm_ensemble_create.argtypes = [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]
+    lib.lbm_ensemble_create.restype = _i
+    lib.lbm_ensemble_destroy.argtypes = [_vp]
+    lib.lbm_ensemble_destroy.restype = None
+    lib.lbm_ensemble_set_schedule.argtypes = [_vp, _i, _i]
+    lib.lbm_ensemble_set_schedule.restype = _i
+    lib.lbm_ensemble_tune.argtypes = [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]
+    lib.lbm_ensemble_tune.restype = _i
+    lib.lbm_ensemble_step.argtypes = [_vp, _i]
+    lib.lbm_ensemble_step.restype = _i
+    lib.lbm_ensemble_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]
+    lib.lbm_ensemble_stats.restype = _i
+    lib.lbm_ensemble_last_error.argtypes = [_vp]
+    lib.lbm_ensemble_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+class EnsembleError(RuntimeError):
+    pass
+
+
+class Ensemble:
+    """Solvers (initialised, same device, no UpdateBCfunc callbacks, no chemistry operator)
+    stepped together.  Fields read through the solvers are current after run()."""
+
+    def __init__(self, solvers, lib=None):
+        self.solvers = list(solvers)
+        clib = lib or self.solvers[0]._lib
+        self.lib = _bind(clib.lib)
+        for lb in self.solvers:
+            if lb._handle is None:
+                raise EnsembleError("init_simulation() every member first")
+            if lb.UpdateBCfunc:
+                raise EnsembleError("members with per-step boundary callbacks cannot be captured")
+            lb._flush()
+            lb._push_params()
+        arr = (_vp * len(self.solvers))(*[lb._handle.value if isinstance(lb._handle, _vp) else lb._handle
+                                         for lb in self.solvers])
+        h = _vp()
+        rc = self.lib.lbm_ensemble_create(arr, len(self.solvers), ctypes.byref(h))
+        if rc != 0:
+            raise EnsembleError(f"lbm_ensemble_create failed (status {rc})")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise EnsembleError(f"{what}: {self.lib.lbm_ensemble_last_error(self.h).decode(errors='replace')} "
+                                f"(status {rc})")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, lanes, pairs_per_graph):
+        self._check(self.lib.lbm_ensemble_set_schedule(self.h, int(lanes), int(pairs_per_graph)), "set_schedule")
+
+    def tune(self, trial_pairs=4):
+        """Autotune lanes x pairs-per-graph on the device (advances the members).  Returns
+        (lanes, pairs_per_graph, {(lanes, pairs): ms per step pair})."""
+        lanes, ppg = _i(), _i()
+        ms = (ctypes.c_double * 10)()
+        self._check(self.lib.lbm_ensemble_tune(self.h, int(trial_pairs), ctypes.byref(lanes), ctypes.byref(ppg), ms),
+                    "tune")
+        table = {}
+        for a, l in enumerate(LANE_CANDIDATES):
+            for b, p in enumerate(PAIR_CANDIDATES):
+                if ms[2 * a + b] >= 0:
+                    table[(l, p)] = ms[2 * a + b]
+        self._sync_counters()
+        return lanes.value, ppg.value, table
+
+    def run(self, nsteps):
+        self._check(self.lib.lbm_ensemble_step(self.h, int(nsteps)), "step")
+        self._sync_counters()
+
+    def _sync_counters(self):
+        for lb in self.solvers:
+            lb.tLattice = lb._lib.steps_done(lb._handle)
+
+    def stats(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self.lib.lbm_ensemble_stats(self.h, ctypes.byref(ms), ctypes.byref(n)), "stats")
+        return {"ms": ms.value, "graph_launches": n.value}
+
+    def cells(self):
+        return sum(lb.nx * lb.ny * lb.nz for lb in self.solvers)
+
+
+def sensitivity_samples(n=64, seed=7):
+    """Seeded parameter sets of the sensitivity ensemble: half 2D (128^2-512^2), half 3D
+    (64^3-128^3); porosity U(0.35, 0.8) (solid fraction 1 - porosity in the particles); inlet
+    temperature U(0.6, 1.2) in lattice units of the 300-800 K range; activation energies
+    scaled by U(0.85, 1.15)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        dim = 2 if i % 2 == 0 else 3
+        if dim == 2:
+            shape = (int(rng.integers(128, 513)), int(rng.integers(128, 513)), 1)
+        else:
+            shape = tuple(int(v) for v in rng.integers(64, 129, size=3))
+        out.append({"dim": dim, "shape": shape, "porosity": float(rng.uniform(0.35, 0.8)),
+                    "T_in": float(rng.uniform(0.6, 1.2)), "ea_scale": float(rng.uniform(0.85, 1.15)),
+                    "seed": int(rng.integers(1 << 30))})
+    return out
+
+
+def build_member(sample, lib=None, precision="fp32", collision="mrt"):
+    """A coupled pyrolysis solver (flow, fluid/solid temperature, 4 species, biomass and char,
+    three Arrhenius routes) for one ensemble sample."""
+    from . import workloads
+    nx, ny, nz = sample["shape"]
+    if sample["dim"] == 3:
+        lb = workloads.coupled3d(lib=lib, nx=nx, ny=ny, nz=nz, seed=sample["seed"], precision=precision,
+                                 collision=collision, solid_value=1.0 - sample["porosity"],
+                                 T_in=300.0 + 500.0 * sample["T_in"], ea_scale=sample["ea_scale"])
+    else:
+        lb = workloads.coupled2d(lib=lib, nx=nx, ny=ny, seed=sample["seed"], precision=precision,
+                                 solid_value=1.0 - sample["porosity"], T_in=300.0 + 500.0 * sample["T_in"],
+                                 ea_scale=sample["ea_scale"])
+    return lb

@@ -105,7 +105,8 @@ SPECIES_3D = (("volatiles", False, 30.0), ("tar", False, 110.0), ("gas", False, 
               ("biomass", True, 162.0), ("char", True, 12.0))
 
 
-def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", collision="mrt"):
+def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", collision="mrt", solid_value=0.5,
+              T_in=800.0, ea_scale=1.0):
     """3D coupled case: D3Q19 flow (MRT with the d'Humieres 2002 non-hydrodynamic rates
     by default, or BGK), D3Q7 fluid/solid temperature and 4 species (volatiles, tar,
     gas, inert) plus FIX biomass/char; gray spheres (radius 6-12 lu, solid fraction
@@ -123,7 +124,7 @@ def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", col
     lb.set_viscosity(1e-5)
     if collision == "mrt":
         lb.set_mrt_standard_rates()
-    solid, mask = gray_packing((nx, ny, nz), 6.0, 12.0, 0.5, seed, 0.5)
+    solid, mask = gray_packing((nx, ny, nz), 6.0, 12.0, 0.5, seed, solid_value)
     lb.add_solid(solid)
     lb.set_poro_Darcy(mask * 1.0, 2e6)
     lb.set_heat_exchange_surface(mask * 1.0)
@@ -141,9 +142,9 @@ def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", col
         lb.set_specie_enthalpy(name, 0.0 if name != "biomass" else -1.0e4)
         lb.set_specie_capacity(name, 30.0)
     kin = pyrolysis_kinetics(seed)
-    lb.add_reaction("biomass => tar", kin[0][0], kin[0][1])
-    lb.add_reaction("biomass => gas", kin[1][0], kin[1][1])
-    lb.add_reaction("biomass => 0.35 char + 0.65 volatiles", kin[2][0], kin[2][1])
+    lb.add_reaction("biomass => tar", kin[0][0], kin[0][1] * ea_scale)
+    lb.add_reaction("biomass => gas", kin[1][0], kin[1][1] * ea_scale)
+    lb.add_reaction("biomass => 0.35 char + 0.65 volatiles", kin[2][0], kin[2][1] * ea_scale)
     lb.init_specie("inert", 1.0)
     lb.init_specie("biomass", mask * 1.0)
     lb.set_BC(0, BC_FLOW.inlet)
@@ -153,9 +154,63 @@ def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", col
     for f in range(2, 6):
         lb.set_BC(f, BC_FLOW.wall)
     lb.set_TF_BC(0, BC.fixedValue)
-    lb.set_TF_BC_value(0, 800.0)
+    lb.set_TF_BC_value(0, T_in)
     lb.set_TS_BC(0, BC.fixedValue)
-    lb.set_TS_BC_value(0, 800.0)
+    lb.set_TS_BC_value(0, T_in)
+    lb.set_TF_BC(1, BC.zeroGradient)
+    lb.set_species_BC(1, BC.zeroGradient)
+    for name, fix, _ in SPECIES_3D:
+        if not fix:
+            lb.set_specie_BC(name, 0, BC.fixedValue)
+            lb.set_specie_BC_value(name, 0, 1.0 if name == "inert" else 0.0)
+    lb.init_field(lb.rho, 1.0)
+    return lb
+
+
+def coupled2d(lib=None, nx=256, ny=256, seed=2025, precision="fp32", solid_value=0.5, T_in=800.0, ea_scale=1.0):
+    """2D twin of coupled3d: D2Q9 BGK flow, D2Q5 fluid/solid temperature and 4 species plus FIX
+    biomass/char, gray disks (radius 4-8 lu), three competing pyrolysis reactions, heated inlet."""
+    from .solver import LBM2DSolver
+    dx = dt = 1e-4
+    lb = LBM2DSolver(nx * dx, ny * dx, 1, dx=dx, dt=dt, name="coupled2d", isThermal=True, isChemical=True,
+                     isPoro=True, isRadiation=False, precision=precision, lib=lib)
+    assert (lb.nx, lb.ny) == (nx, ny), (lb.nx, lb.ny)
+    lb.source_term_model = SOURCE_TERM.MICRO
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(1e-5)
+    solid, mask = gray_packing((nx, ny, 1), 4.0, 8.0, 0.5, seed, solid_value)
+    lb.add_solid(solid)
+    lb.set_poro_Darcy(mask * 1.0, 2e6)
+    lb.set_heat_exchange_surface(mask * 1.0)
+    lb.set_heat_exchange_coef(mask * 5e4)
+    lb.set_fluid_Trange([300.0, 800.0])
+    lb.set_solid_Trange([300.0, 800.0])
+    lb.set_fluid_Prandtl(0.71)
+    lb.set_fluid_capacity(1100.0)
+    lb.set_solid_thermal_diff(2e-6)
+    lb.set_solid_capacity(1500.0)
+    for name, fix, mm in SPECIES_3D:
+        lb.set_specie(name, fix, mm)
+        if not fix:
+            lb.set_specie_diff(name, 2e-5)
+        lb.set_specie_enthalpy(name, 0.0 if name != "biomass" else -1.0e4)
+        lb.set_specie_capacity(name, 30.0)
+    kin = pyrolysis_kinetics(seed)
+    lb.add_reaction("biomass => tar", kin[0][0], kin[0][1] * ea_scale)
+    lb.add_reaction("biomass => gas", kin[1][0], kin[1][1] * ea_scale)
+    lb.add_reaction("biomass => 0.35 char + 0.65 volatiles", kin[2][0], kin[2][1] * ea_scale)
+    lb.init_specie("inert", 1.0)
+    lb.init_specie("biomass", mask * 1.0)
+    lb.set_BC(0, BC_FLOW.inlet)
+    lb.set_v_BC_value(0, [0.02, 0.0, 0.0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    lb.set_BC(2, BC_FLOW.wall)
+    lb.set_BC(3, BC_FLOW.wall)
+    lb.set_TF_BC(0, BC.fixedValue)
+    lb.set_TF_BC_value(0, T_in)
+    lb.set_TS_BC(0, BC.fixedValue)
+    lb.set_TS_BC_value(0, T_in)
     lb.set_TF_BC(1, BC.zeroGradient)
     lb.set_species_BC(1, BC.zeroGradient)
     for name, fix, _ in SPECIES_3D:
@@ -249,3 +304,4 @@ COUPLED3D_BYTES = {
 PLAIN2D_BYTES_FP64 = bytes_per_cell(9 + 5 + 5, 8, 3, 0, 8)  # 328
 PYRO12_Q = 19 + 7 + 7 + 12 * 7  # 117
 PYRO12_BYTES_FP32 = bytes_per_cell(PYRO12_Q, 4, 3, 3, 4)  # 972
+COUPLED2D_BYTES_FP32 = bytes_per_cell(9 + 5 + 5 + 4 * 5, 4, 3, 3, 4)  # 348

@@ -235,9 +235,9 @@ int chem_oracle_cell(const lbm_chem_mech* m, const double* in, double dt, double
     atol[NS + 1] = m->atol_B;
 
     double t = 0.0, h = dt;
-    int nacc = 0, natt = 0, rejected = 0, ok = 1;
+    int nacc = 0, natt = 0, rejected = 0, ok = 1, age = 0, have_jac = 0;
     tcache tc;
-    double f0[NY], f1[NY], k1[NY], k2[NY], y1[NY], yn[NY], J[NY][NY];
+    double f0[NY], f1[NY], k1[NY], k2[NY], y1[NY], yn[NY], Jac[NY][NY], J[NY][NY];
     int piv[NY];
     for (;;) {
         const int last = h >= dt - t;
@@ -249,9 +249,13 @@ int chem_oracle_cell(const lbm_chem_mech* m, const double* in, double dt, double
         ++natt;
         tcache_fill(m, y[NS], &tc);
         rhs(m, &tc, Csol, y, f0);
-        jacobian(m, &tc, Csol, y, f0, J);
+        if (!have_jac || rejected || age >= m->jac_max_age) {
+            jacobian(m, &tc, Csol, y, f0, Jac);
+            have_jac = 1;
+            age = 0;
+        }
         for (int i = 0; i < NY; ++i)
-            for (int j = 0; j < NY; ++j) J[i][j] = (i == j ? 1.0 : 0.0) - gam * h * J[i][j];
+            for (int j = 0; j < NY; ++j) J[i][j] = (i == j ? 1.0 : 0.0) - gam * h * Jac[i][j];
         lu_factor(J, piv);
         for (int i = 0; i < NY; ++i) k1[i] = f0[i];
         lu_solve(J, piv, k1);
@@ -274,6 +278,7 @@ int chem_oracle_cell(const lbm_chem_mech* m, const double* in, double dt, double
         if (err <= 1.0) {
             for (int i = 0; i < NY; ++i) y[i] = yn[i];
             ++nacc;
+            ++age;
             if (last) break;
             t += h;
             if (fac > 5.0) fac = 5.0;

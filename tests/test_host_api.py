@@ -54,6 +54,31 @@ def test_python_binding_names_every_symbol():
     assert set(chem._CHEM_SYMBOLS) == set(declared_functions(HEADERS[1:]))
 
 
+def test_product_exports_ensemble_entry_points():
+    """lbm_ensemble.h is served by the product library only (graph capture has no CPU analogue)."""
+    from lbm_porouschem_amd import ensemble
+    header = os.path.join(ROOT, "include", "lbm_ensemble.h")
+    text = re.sub(r"/\*.*?\*/", "", open(header).read(), flags=re.S)
+    names = sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(lbm_\w+)\s*\(", text, flags=re.M)))
+    assert set(names) == set(ensemble._ENS_SYMBOLS)
+    dyn = subprocess.run(["nm", "-D", "--defined-only", _abi.library_path()], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT\s+(lbm_\w+)$", dyn, flags=re.M))
+    assert set(names) <= exported, set(names) - exported
+
+
+def test_sensitivity_samples_cover_the_ranges():
+    from lbm_porouschem_amd import ensemble
+    s = ensemble.sensitivity_samples(64, seed=7)
+    assert len(s) == 64 and sum(x["dim"] == 2 for x in s) == 32
+    for x in s:
+        if x["dim"] == 2:
+            assert 128 <= x["shape"][0] <= 512 and 128 <= x["shape"][1] <= 512 and x["shape"][2] == 1
+        else:
+            assert all(64 <= v <= 128 for v in x["shape"])
+        assert 0.35 <= x["porosity"] <= 0.8 and 0.6 <= x["T_in"] <= 1.2 and 0.85 <= x["ea_scale"] <= 1.15
+
+
 @pytest.mark.parametrize("which", sorted(libraries()))
 def test_chemistry_struct_layout(which, oracle64):
     from lbm_porouschem_amd import chem
