@@ -81,13 +81,13 @@ def init_params(seed=0):
 
 def normalisation(X, Y, margin=0.02):
     """Per-feature z-score of inputs and outputs; hull = training-input bounding box widened
-    by `margin` of its range."""
+    by `margin` of its range.  An output that is constant over the data (the change of an
+    inert species) gets scale 0: the surrogate then returns that constant exactly."""
     in_mean = X.mean(axis=1)
     in_sd = X.std(axis=1)
     in_sd[in_sd == 0] = 1.0
     out_mean = Y.mean(axis=1)
     out_sd = Y.std(axis=1)
-    out_sd[out_sd == 0] = 1.0
     lo, hi = X.min(axis=1), X.max(axis=1)
     span = hi - lo
     return {
@@ -99,7 +99,8 @@ def normalisation(X, Y, margin=0.02):
 
 def normalise(X, Y, norm):
     Xn = ((X.astype(np.float32) - norm["in_mean"][:, None]) * norm["in_scale"][:, None]).astype(np.float32)
-    Yn = ((Y - norm["out_mean"][:, None]) / norm["out_scale"][:, None]).astype(np.float32)
+    sd = norm["out_scale"].astype(np.float64)
+    Yn = ((Y - norm["out_mean"][:, None]) / np.where(sd > 0, sd, 1.0)[:, None]).astype(np.float32)
     return Xn, Yn
 
 

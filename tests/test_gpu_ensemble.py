@@ -63,8 +63,9 @@ def test_ensemble_tune_and_member_eager_step(hip):
     ref = members(hip)
     ens_m = members(hip)
     ens = ensemble.Ensemble(ens_m)
-    lanes, ppg, table = ens.tune(trial_pairs=2)
-    assert (lanes, ppg) in table and len(table) >= 4
+    lanes, ppg, table = ens.tune(trial_pairs=4)
+    # 5 members: lanes 1, 2, 4 x pairs per graph 1, 4
+    assert (lanes, ppg) in table and len(table) == 6
     advanced = ens_m[0].tLattice
     assert all(lb.tLattice == advanced for lb in ens_m) and advanced % 2 == 0
     for lb in ref:
