@@ -86,6 +86,69 @@ def cpu_baseline(size, collision):
     }
 
 
+SURROGATE_FILE = os.path.join(ROOT, "lbm_porouschem_amd", "chem", "syngas12_surrogate.npz")
+MLP_FLOP_PER_CELL = 2 * (16 * 128 + 128 * 128 + 128 * 14)  # 40448
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_leg(lib, n, steps):
+    """Coupled pyrolysis with detailed gas-phase kinetics (117 populations per cell) at n^3 FP32:
+    the same steps from the same initial state with the plain FP64 stiff integrator and with
+    the committed MLP surrogate (per-cell fallback on).  Returns per-step times, the speedup,
+    the fields' difference after those steps and the surrogate kernel's matrix-core rate."""
+    import numpy as np
+    from lbm_porouschem_amd import chem
+    from lbm_porouschem_amd.chem import surrogate as S
+    if not os.path.exists(SURROGATE_FILE):
+        return None
+    sur = S.load(SURROGATE_FILE)
+    out = {"workload": f"pyrolysis12 {n}x{n}x{n} fp32 MRT: D3Q19 + D3Q7 TF + D3Q7 TS + 12 D3Q7 gas species "
+                       "(GRI-Mech 3.0 H2/CO subset, 40 reactions) + biomass/char, 3 pyrolysis routes",
+           "steps": steps}
+    fields = {}
+    for name, mode in (("plain", chem.MODE_PLAIN), ("surrogate", chem.MODE_SURROGATE)):
+        lb = workloads.pyrolysis12(lib=lib, nx=n, ny=n, nz=n)
+        lb.init_simulation()
+        g = chem.Chem(lib)
+        if mode == chem.MODE_SURROGATE:
+            g.set_surrogate(sur)
+        lb.attach_chemistry(g, mode, rho_solid_ref=lb.chem_rho_solid_ref)
+        b0 = lb.species[12].S.to_numpy().astype(np.float64).sum()
+        lb.run(1)
+        lb.sync()
+        h = lb._handle
+        lib.profile(h, 1)
+        t0 = time.perf_counter()
+        lb.run(steps)
+        lb.sync()
+        wall = time.perf_counter() - t0
+        ms_chem, n_chem = lib.profile_read(h, 4)
+        ms_bulk, n_bulk = lib.profile_read(h, _abi.KERNEL_BULK)
+        lib.profile(h, 0)
+        st = lb.chemistry_stats(g)
+        b1 = lb.species[12].S.to_numpy().astype(np.float64).sum()
+        fields[name] = (lb.TF.get_physical_value(lb.TF.S.to_numpy()), lb.TS.get_physical_value(lb.TS.S.to_numpy()),
+                        1.0 - b1 / b0)
+        out[name] = {"ms_per_step": round(wall / steps * 1e3, 3), "chem_ms_per_step": round(ms_chem / n_chem, 3),
+                     "bulk_ms_per_step": round(ms_bulk / n_bulk, 3), "fallback_hull": st["fallback_hull"],
+                     "fallback_residual": st["fallback_residual"], "cells_evaluated": st["cells"]}
+        log(f"[bench] surrogate leg {name}: {out[name]}")
+        del lb
+    (tfp, tsp, cp), (tfs, tss, cs) = fields["plain"], fields["surrogate"]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["speedup"] = round(out["plain"]["ms_per_step"] / out["surrogate"]["ms_per_step"], 2)
+    out["rel_l2_TF"] = rel(tfs, tfp)
+    out["rel_l2_TS"] = rel(tss, tsp)
+    out["conversion_plain"] = cp
+    out["rel_err_conversion"] = abs(cs - cp) / abs(cp) if cp else None
+    cells = out["surrogate"]["cells_evaluated"] / (steps + 1)
+    tflops = MLP_FLOP_PER_CELL * cells / (out["surrogate"]["chem_ms_per_step"] / 1e3) / 1e12
+    out["mfma"] = {"bound": "mfma", "achieved": round(tflops, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "frac": round(tflops / BF16_PEAK_TFLOPS, 4),
+                   "note": "operator pass time (surrogate kernel + fallback integration) per step"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +158,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "fp64"])
     ap.add_argument("--collision", default="mrt", choices=["mrt", "bgk"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=3,
+                    help="steps of the plain-vs-surrogate chemistry comparison (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this many MiB (scale reference for rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -195,9 +260,11 @@ def main():
         },
         "cpu_baseline": None,
     }
+    del lb
+    if args.surrogate_steps > 0:
+        result["surrogate"] = surrogate_leg(lib, n, args.surrogate_steps)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (bounded sample)")
-        del lb
         result["cpu_baseline"] = cpu_baseline((n, n // 2, n // 2), args.collision)
     print(json.dumps(result), flush=True)
 

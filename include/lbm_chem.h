@@ -94,16 +94,19 @@ typedef struct lbm_chem_mech {
 } lbm_chem_mech;
 
 /* Surrogate: x_n = (x - in_mean) * in_scale per input, y = y_n * out_scale + out_mean per
- * output; layers h1 = relu(W1 x_n + b1), h2 = relu(W2 h1 + b2), y_n = W3 h2 + b3 with BF16
- * weights and layer inputs, FP32 accumulation.  A cell falls back to the plain integrator
- * when any raw input leaves [hull_lo, hull_hi] or when the element-balance residual of
- * the prediction exceeds residual_tol (lbm_chem_residual). */
+ * output, then y *= x[out_mul[o]] where out_mul[o] >= 0 (an output predicted per unit of an
+ * input, e.g. the biomass change per unit of biomass); layers h1 = relu(W1 x_n + b1),
+ * h2 = relu(W2 h1 + b2), y_n = W3 h2 + b3 with BF16 weights and layer inputs, FP32
+ * accumulation.  A cell falls back to the plain integrator when any raw input leaves
+ * [hull_lo, hull_hi] or when the element-balance residual of the prediction exceeds
+ * residual_tol (lbm_chem_residual). */
 typedef struct lbm_surrogate {
     const float* params;                   /* host, LBM_MLP_NPARAM FP32 values */
     float in_mean[LBM_CHEM_NIN], in_scale[LBM_CHEM_NIN];
     float out_mean[LBM_CHEM_NOUT], out_scale[LBM_CHEM_NOUT];
     float hull_lo[LBM_CHEM_NIN], hull_hi[LBM_CHEM_NIN];
     double residual_tol;
+    int out_mul[LBM_CHEM_NOUT];            /* input index multiplying each output, -1: none */
 } lbm_surrogate;
 
 typedef struct lbm_chem lbm_chem;

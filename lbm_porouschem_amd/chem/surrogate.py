@@ -34,14 +34,14 @@ def cell_inputs_from_solver(lb, p0=101325.0, rho_solid_ref=2.0):
 
 
 def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000, seed=0, precision="fp32",
-            log=print):
+            packing_seed=2025, log=print):
     """Run the coupled pyrolysis case with the plain operator attached and log (inputs, outputs)
     of the plain FP64 integrator every `every` steps.  Returns (X [16, n], Y [14, n], dt)."""
     import torch
 
     from .. import workloads
     nx, ny, nz = shape
-    lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision)
+    lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision, seed=packing_seed)
     lb.init_simulation()
     g = Chem(lib)
     lb.attach_chemistry(g, MODE_PLAIN, rho_solid_ref=lb.chem_rho_solid_ref)
@@ -97,6 +97,21 @@ def normalisation(X, Y, margin=0.02):
     }
 
 
+# Output 12 (biomass change) is learned per unit of biomass (input 14): a function of the
+# temperature and composition only, exactly zero where there is no biomass.
+OUT_MUL = [-1] * 12 + [14, -1]
+
+
+def per_unit_targets(X, Y, out_mul=OUT_MUL):
+    """Training targets of outputs predicted per unit of an input (0 where that input is 0)."""
+    T = np.array(Y, dtype=np.float64, copy=True)
+    for o, v in enumerate(out_mul):
+        if v >= 0:
+            d = X[v]
+            T[o] = np.divide(Y[o], d, out=np.zeros_like(Y[o]), where=d > 0)
+    return T
+
+
 def normalise(X, Y, norm):
     Xn = ((X.astype(np.float32) - norm["in_mean"][:, None]) * norm["in_scale"][:, None]).astype(np.float32)
     sd = norm["out_scale"].astype(np.float64)
@@ -104,17 +119,18 @@ def normalise(X, Y, norm):
     return Xn, Yn
 
 
-def train(g, X, Y, epochs=20, batch=8192, lr=2e-3, lr_final=1e-4, holdout=0.05, residual_tol=1e-3, seed=0,
-          params=None, log=print):
+def train(g, X, Y, epochs=20, batch=8192, lr=1e-3, lr_final=5e-5, holdout=0.05, residual_tol=1e-3, seed=0,
+          params=None, margin=0.05, log=print):
     """Fit the 16-128-128-14 MLP on the device with the library's kernels (lbm_mlp_grad +
     lbm_mlp_adam), cosine learning-rate decay.  Returns (surrogate dict, history)."""
     import torch
     rng = np.random.default_rng(seed)
+    Y = per_unit_targets(X, Y)
     n = X.shape[1]
     perm = rng.permutation(n)
     nh = max(int(n * holdout), 1)
     hold, fit = perm[:nh], perm[nh:]
-    norm = normalisation(X[:, fit], Y[:, fit])
+    norm = normalisation(X[:, fit], Y[:, fit], margin=margin)
     Xn, Yn = normalise(X, Y, norm)
     Xd = torch.from_numpy(np.ascontiguousarray(Xn[:, fit])).cuda()
     Yd = torch.from_numpy(np.ascontiguousarray(Yn[:, fit])).cuda()
@@ -149,6 +165,7 @@ def train(g, X, Y, epochs=20, batch=8192, lr=2e-3, lr_final=1e-4, holdout=0.05, 
     sur = dict(norm)
     sur["params"] = P.cpu().numpy()
     sur["residual_tol"] = residual_tol
+    sur["out_mul"] = np.array(OUT_MUL, dtype=np.int32)
     return sur, hist
 
 

@@ -542,6 +542,7 @@ struct SurDev {
     float b1[HID], b2[HID], b3[16];
     float in_mean[16], in_scale[16], out_mean[16], out_scale[16], lo[16], hi[16];
     double residual_tol;
+    int out_mul[16];
 };
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -674,7 +675,11 @@ __global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restric
             if constexpr (CHECK) {
                 double y[NO];
 #pragma unroll
-                for (int o = 0; o < NO; ++o) y[o] = (double)(ys[wv][l * 16 + o] * S->out_scale[o] + S->out_mean[o]);
+                for (int o = 0; o < NO; ++o) {
+                    y[o] = (double)(ys[wv][l * 16 + o] * S->out_scale[o] + S->out_mean[o]);
+                    const int mv = S->out_mul[o];
+                    if (mv >= 0) y[o] *= x[mv];
+                }
 #pragma unroll
                 for (int v = 0; v < NI; ++v)
                     if (x[v] < (double)S->lo[v] || x[v] > (double)S->hi[v]) hull = true;
@@ -1004,6 +1009,8 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
         return LBM_OK;
     }
     if (!s->params) return cerr(c, LBM_ERR_INVALID, "surrogate without parameters");
+    for (int o = 0; o < NO; ++o)
+        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI) return cerr(c, LBM_ERR_INVALID, "out_mul[%d] = %d", o, s->out_mul[o]);
     CHEM_TRY(hipSetDevice(c->device));
     CHEM_TRY(hipMemcpyAsync(c->d_params, s->params, NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_pack, dim3((NTILE * 256 + 255) / 256), dim3(256), 0, c->stream, c->d_params, c->d_sur);
@@ -1020,6 +1027,7 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
         host_tail.out_scale[i] = i < NO ? s->out_scale[i] : 0.0f;
     }
     host_tail.residual_tol = s->residual_tol;
+    for (int o = 0; o < 16; ++o) host_tail.out_mul[o] = o < NO ? s->out_mul[o] : -1;
     const size_t off = offsetof(SurDev, in_mean);
     CHEM_TRY(hipMemcpyAsync((char*)c->d_sur + off, (const char*)&host_tail + off, sizeof(SurDev) - off,
                             hipMemcpyHostToDevice, c->stream));

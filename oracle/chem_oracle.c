@@ -481,6 +481,8 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
         return LBM_OK;
     }
     if (!s->params) return LBM_ERR_INVALID;
+    for (int o = 0; o < NO; ++o)
+        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI) return LBM_ERR_INVALID;
     memcpy(c->params, s->params, sizeof(c->params));
     c->sur = *s;
     c->sur.params = c->params;
@@ -498,7 +500,10 @@ int chem_oracle_surrogate_cell(const lbm_chem* c, const double* in, double dt, i
     }
     mlp_trace t;
     mlp_fwd(c->params, xn, &t);
-    for (int o = 0; o < NO; ++o) out[o] = (double)((float)t.y[o] * s->out_scale[o] + s->out_mean[o]);
+    for (int o = 0; o < NO; ++o) {
+        out[o] = (double)((float)t.y[o] * s->out_scale[o] + s->out_mean[o]);
+        if (s->out_mul[o] >= 0) out[o] *= in[s->out_mul[o]];
+    }
     if (!flag && lbm_chem_residual(&c->mech, in, out) > s->residual_tol) flag = 2;
     if (flag && fallback) chem_oracle_cell(&c->mech, in, dt, out);
     return flag;
