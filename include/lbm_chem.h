@@ -28,7 +28,9 @@
  * The char change follows from mass conservation, d rho_C = -d rho_B - sum_k d rho_k.
  *
  * Array arguments of the device entry points are device pointers, structure of
- * arrays: value v of cell i at [v * n + i].
+ * arrays: value v of cell i at [v * n + i].  The context's kernels start after all work
+ * queued on the device's default stream and the entry points return after they complete;
+ * a caller that writes the arrays on another stream synchronises that stream first.
  */
 #ifndef LBM_CHEM_H
 #define LBM_CHEM_H

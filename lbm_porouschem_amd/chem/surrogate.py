@@ -33,13 +33,43 @@ def cell_inputs_from_solver(lb, p0=101325.0, rho_solid_ref=2.0):
     return x
 
 
+MAJOR = ("H2", "O2", "H2O", "CO", "CO2")
+
+
+def jitter(x, rng):
+    """A perturbed copy of logged cell states: T +-8 K, p +-2 %, trace species x U(0, 2), major
+    species x U(0.9, 1.1), N2 closing the mass fractions, biomass x U(0.9, 1), char x U(0.8, 1.2).
+    Training on these as well teaches the network the neighbourhood of the trajectories, which
+    is where a coupled run with the surrogate's own small errors goes."""
+    from . import species_names
+    names = species_names()
+    n = x.shape[1]
+    xa = x.copy()
+    xa[0] += rng.uniform(-8.0, 8.0, n)
+    xa[1] *= rng.uniform(0.98, 1.02, n)
+    for k, name in enumerate(names):
+        if name == "N2":
+            continue
+        if name in MAJOR:
+            xa[2 + k] = np.maximum(xa[2 + k] * rng.uniform(0.9, 1.1, n), 0.0)
+        else:
+            xa[2 + k] *= rng.uniform(0.0, 2.0, n)
+    k2 = names.index("N2")
+    xa[2 + k2] = 1.0 - (xa[2:14].sum(axis=0) - xa[2 + k2])
+    xa[14] *= rng.uniform(0.9, 1.0, n)
+    xa[15] *= rng.uniform(0.8, 1.2, n)
+    return xa
+
+
 def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000, seed=0, precision="fp32",
-            packing_seed=2025, log=print):
+            packing_seed=2025, jittered=True, log=print):
     """Run the coupled pyrolysis case with the plain operator attached and log (inputs, outputs)
-    of the plain FP64 integrator every `every` steps.  Returns (X [16, n], Y [14, n], dt)."""
+    of the plain FP64 integrator every `every` steps, plus (jittered=True) the integrator's
+    outputs on a perturbed copy of each snapshot.  Returns (X [16, n], Y [14, n], dt)."""
     import torch
 
     from .. import workloads
+    rng = np.random.default_rng(seed + packing_seed)
     nx, ny, nz = shape
     lb = workloads.pyrolysis12(lib=lib, nx=nx, ny=ny, nz=nz, precision=precision, seed=packing_seed)
     lb.init_simulation()
@@ -50,16 +80,17 @@ def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000,
     for s in range(0, steps + 1, every):
         if s:
             lb.run(every)
-        x = cell_inputs_from_solver(lb, rho_solid_ref=lb.chem_rho_solid_ref)
-        xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
-        yd = torch.zeros((NOUT, x.shape[1]), dtype=torch.float64, device="cuda")
-        nd = torch.zeros(x.shape[1], dtype=torch.int32, device="cuda")
-        g.integrate(xd, yd, nd, dt=lb.dt)
-        xs.append(x)
-        ys.append(yd.cpu().numpy())
+        x0 = cell_inputs_from_solver(lb, rho_solid_ref=lb.chem_rho_solid_ref)
+        for x in ((x0, jitter(x0, rng)) if jittered else (x0,)):
+            xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+            yd = torch.zeros((NOUT, x.shape[1]), dtype=torch.float64, device="cuda")
+            nd = torch.zeros(x.shape[1], dtype=torch.int32, device="cuda")
+            g.integrate(xd, yd, nd, dt=lb.dt)
+            xs.append(x)
+            ys.append(yd.cpu().numpy())
         ns = nd.cpu().numpy()
-        log(f"collect: step {s}: {x.shape[1]} cells, T {x[0].min():.0f}-{x[0].max():.0f} K, integrator steps "
-            f"mean {ns.mean():.1f} max {ns.max()} ({time.perf_counter() - t0:.1f} s)")
+        log(f"collect: packing {packing_seed} step {s}: {x0.shape[1]} cells, T {x0[0].min():.0f}-{x0[0].max():.0f} K, "
+            f"integrator steps mean {ns.mean():.1f} max {ns.max()} ({time.perf_counter() - t0:.1f} s)")
     X = np.concatenate(xs, axis=1)
     Y = np.concatenate(ys, axis=1)
     if X.shape[1] > max_samples:
@@ -119,10 +150,16 @@ def normalise(X, Y, norm):
     return Xn, Yn
 
 
-def train(g, X, Y, epochs=20, batch=8192, lr=1e-3, lr_final=5e-5, holdout=0.05, residual_tol=1e-3, seed=0,
-          params=None, margin=0.05, log=print):
+# loss weight (as a divisor of the output scale, so the weight is its square) of the outputs
+# that the coupled run integrates over many steps: the biomass change and the temperature change
+OUT_EMPHASIS = {12: 3.0, 13: 3.0}
+
+
+def train(g, X, Y, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0.05, residual_tol=1e-3, seed=0,
+          params=None, margin=0.5, emphasis=OUT_EMPHASIS, log=print):
     """Fit the 16-128-128-14 MLP on the device with the library's kernels (lbm_mlp_grad +
-    lbm_mlp_adam), cosine learning-rate decay.  Returns (surrogate dict, history)."""
+    lbm_mlp_adam); the learning rate decays by `decay` per epoch.  Returns (surrogate dict,
+    history)."""
     import torch
     rng = np.random.default_rng(seed)
     Y = per_unit_targets(X, Y)
@@ -131,6 +168,8 @@ def train(g, X, Y, epochs=20, batch=8192, lr=1e-3, lr_final=5e-5, holdout=0.05, 
     nh = max(int(n * holdout), 1)
     hold, fit = perm[:nh], perm[nh:]
     norm = normalisation(X[:, fit], Y[:, fit], margin=margin)
+    for o, w in (emphasis or {}).items():
+        norm["out_scale"][o] /= w
     Xn, Yn = normalise(X, Y, norm)
     Xd = torch.from_numpy(np.ascontiguousarray(Xn[:, fit])).cuda()
     Yd = torch.from_numpy(np.ascontiguousarray(Yn[:, fit])).cuda()
@@ -142,10 +181,10 @@ def train(g, X, Y, epochs=20, batch=8192, lr=1e-3, lr_final=5e-5, holdout=0.05, 
     V = torch.zeros_like(P)
     nf = Xd.shape[1]
     steps_per_epoch = max(nf // batch, 1)
-    total = epochs * steps_per_epoch
     hist = []
     t = 0
     t0 = time.perf_counter()
+    rate = lr
     for ep in range(epochs):
         order = torch.randperm(nf, device="cuda")
         tot = 0.0
@@ -155,8 +194,8 @@ def train(g, X, Y, epochs=20, batch=8192, lr=1e-3, lr_final=5e-5, holdout=0.05, 
             yb = Yd[:, idx].contiguous()
             tot += g.mlp_grad(P, xb, yb, G)
             t += 1
-            rate = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * t / total))
             g.mlp_adam(P, G, M, V, rate, t)
+        rate *= decay
         hl = holdout_loss(g, P, Xh, Yh)
         hist.append({"epoch": ep, "train_loss": tot / steps_per_epoch, "holdout_loss": hl,
                      "seconds": time.perf_counter() - t0})

@@ -140,9 +140,9 @@ def main():
     ap.add_argument("--train-shape", default="96,64,64")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--every", type=int, default=20)
-    ap.add_argument("--train-seeds", default="2025,2027", help="sphere packings the training data come from")
+    ap.add_argument("--train-seeds", default="2025,2027,2029", help="sphere packings the training data come from")
     ap.add_argument("--samples", type=int, default=2_000_000)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--eval-shape", default="128,64,64")
     ap.add_argument("--eval-steps", type=int, default=400)
     ap.add_argument("--eval-seed", type=int, default=2026, help="sphere packing of the evaluation run")

@@ -853,6 +853,7 @@ struct lbm_chem {
     double* d_loss = nullptr;
     long long loss_cap = 0;
     int n_cu = 256;
+    hipEvent_t ev_dflt = nullptr;
     char err[512] = {0};
 };
 
@@ -883,6 +884,15 @@ int ensure_list(lbm_chem* c, long long n) {
     c->list_cap = 0;
     CHEM_TRY(hipMalloc(&c->d_list, (size_t)(n > 0 ? n : 1) * sizeof(unsigned)));
     c->list_cap = n;
+    return LBM_OK;
+}
+
+// The device entry points borrow arrays that the caller may have just written on the
+// device's default stream (torch's stream unless the caller switched streams): the context's
+// stream waits for everything queued there before its kernels start.
+int after_default_stream(lbm_chem* c) {
+    CHEM_TRY(hipEventRecord(c->ev_dflt, nullptr));
+    CHEM_TRY(hipStreamWaitEvent(c->stream, c->ev_dflt, 0));
     return LBM_OK;
 }
 
@@ -956,6 +966,7 @@ int lbm_chem_create(const lbm_chem_mech* mech, int device, lbm_chem** out) {
         if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
             c->n_cu = prop.multiProcessorCount;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
+        if (hipEventCreateWithFlags(&c->ev_dflt, hipEventDisableTiming) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
         if (hipMalloc(&c->d_mech, sizeof(lbm_chem_mech)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMalloc(&c->d_molemass, NS * sizeof(double)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMalloc(&c->d_sur, sizeof(SurDev)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
@@ -980,6 +991,7 @@ void lbm_chem_destroy(lbm_chem* c) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamDestroy(c->stream);
     }
+    if (c->ev_dflt) (void)hipEventDestroy(c->ev_dflt);
     void* ptrs[] = {c->d_mech, c->d_molemass, c->d_sur, c->d_tmp, c->d_params, c->d_list, c->d_count, c->d_ws, c->d_loss};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -992,6 +1004,7 @@ int lbm_chem_integrate(lbm_chem* c, const double* in, double* out, int* nsteps, 
     if (!c || (!in && n) || (!out && n) || n < 0) return LBM_ERR_INVALID;
     if (n == 0) return LBM_OK;
     CHEM_TRY(hipSetDevice(c->device));
+    if (int rc0 = after_default_stream(c)) return rc0;
     SrcF64 src{in, out, nsteps, n};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->d_count, 2);
     hipLaunchKernelGGL((k_plain<SrcF64>), dim3(grid_plain(c, n)), dim3(PLAIN_BLOCK), 0, c->stream, c->d_mech, src, n,
@@ -1044,6 +1057,7 @@ int lbm_chem_surrogate(lbm_chem* c, const float* in, float* out, int* flags, lon
     CHEM_TRY(hipSetDevice(c->device));
     int rc = ensure_list(c, n);
     if (rc) return rc;
+    if (int rc0 = after_default_stream(c)) return rc0;
     SrcF32 src{in, out, flags, n, fallback, c->d_list, c->d_count};
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->d_count, 2);
     hipLaunchKernelGGL((k_surrogate<SrcF32, 1>), dim3(grid_tiles(c, n)), dim3(SUR_BLOCK), 0, c->stream, c->d_sur,
@@ -1062,6 +1076,7 @@ int lbm_mlp_forward(lbm_chem* c, const float* params, const float* xn, float* yn
     if (!c || !params || n < 0 || (n && (!xn || !yn))) return LBM_ERR_INVALID;
     if (n == 0) return LBM_OK;
     CHEM_TRY(hipSetDevice(c->device));
+    if (int rc0 = after_default_stream(c)) return rc0;
     hipLaunchKernelGGL(k_pack, dim3((NTILE * 256 + 255) / 256), dim3(256), 0, c->stream, params, c->d_tmp);
     SrcNorm src{xn, yn, n};
     hipLaunchKernelGGL((k_surrogate<SrcNorm, 0>), dim3(grid_tiles(c, n)), dim3(SUR_BLOCK), 0, c->stream, c->d_tmp,
@@ -1090,6 +1105,7 @@ int lbm_mlp_grad(lbm_chem* c, const float* params, const float* xn, const float*
         CHEM_TRY(hipMalloc(&c->d_loss, (size_t)nblk * sizeof(double)));
         c->loss_cap = nblk;
     }
+    if (int rc0 = after_default_stream(c)) return rc0;
     hipLaunchKernelGGL(k_train, dim3((unsigned)nblk), dim3(TRAIN_BLOCK), 0, c->stream, params, xn, yn_target, n,
                        c->d_ws, c->d_loss);
     hipLaunchKernelGGL(k_wgrad, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, xn, c->d_ws, n, grad);
@@ -1110,6 +1126,7 @@ int lbm_mlp_grad(lbm_chem* c, const float* params, const float* xn, const float*
 int lbm_mlp_adam(lbm_chem* c, float* params, const float* grad, float* m, float* v, float lr, int t) {
     if (!c || !params || !grad || !m || !v || t < 1) return LBM_ERR_INVALID;
     CHEM_TRY(hipSetDevice(c->device));
+    if (int rc0 = after_default_stream(c)) return rc0;
     const float c1 = (float)(1.0 / (1.0 - pow(0.9, t))), c2 = (float)(1.0 / (1.0 - pow(0.999, t)));
     hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, params, grad, m, v, lr, c1, c2);
     CHEM_TRY(hipGetLastError());
