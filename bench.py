@@ -86,6 +86,52 @@ def cpu_baseline(size, collision):
     }
 
 
+def plain2d_leg(lib, steps, cpu_steps):
+    """The 2D plain case (256x128 FP64, D2Q9 BGK + D2Q5 fluid/solid temperature, gray disks)
+    for `steps` steps: eager lbm_step, then the same handle stepped through a one-member
+    ensemble (captured HIP graphs of step pairs).  32,768 cells fill only 128 workgroups, so
+    the step is bound by launch latency, not HBM; the CPU restatement is timed beside it."""
+    from lbm_porouschem_amd import _abi, ensemble, workloads
+    out = {"workload": "plain2d 256x128 fp64: D2Q9 BGK tau 0.8 + D2Q5 TF + D2Q5 TS, gray disks r 4-8 "
+                       "solid 0.35, inlet u 0.02 T 1, outflow, walls", "steps": steps}
+    cells = 256 * 128
+    lb = workloads.plain2d(lib=lib)
+    lb.init_simulation()
+    h = lb._handle
+    lib.step(h, 10)
+    lib.sync(h)
+    t0 = time.perf_counter()
+    lib.step(h, steps)
+    lib.sync(h)
+    wall = time.perf_counter() - t0
+    out["eager"] = {"MLUPS": round(cells * steps / wall / 1e6, 1), "us_per_step": round(wall / steps * 1e6, 2)}
+    ens = ensemble.Ensemble([lb], lib=lib)
+    ens.set_schedule(1, 16)
+    ens.run(64)
+    lib.sync(h)
+    t0 = time.perf_counter()
+    ens.run(steps)
+    lib.sync(h)
+    wall = time.perf_counter() - t0
+    out["graph"] = {"MLUPS": round(cells * steps / wall / 1e6, 1), "us_per_step": round(wall / steps * 1e6, 2),
+                    "pairs_per_graph": 16}
+    ens.close()
+    del lb
+    if cpu_steps:
+        olib = _abi.CLib(os.path.join(ROOT, "oracle", "liblbm_oracle.so"))
+        ob = workloads.plain2d(lib=olib)
+        ob.init_simulation()
+        ob.run(10)
+        t0 = time.perf_counter()
+        ob.run(cpu_steps)
+        wall = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu"] = {"MLUPS": round(cells * cpu_steps / wall / 1e6, 2), "steps": cpu_steps, "cores": threads,
+                      "kind": "port (oracle/lbm_oracle.c FP64, OpenMP)"}
+    log(f"[bench] plain2d leg: {out}")
+    return out
+
+
 SURROGATE_FILE = os.path.join(ROOT, "lbm_porouschem_amd", "chem", "syngas12_surrogate.npz")
 MLP_FLOP_PER_CELL = 2 * (16 * 128 + 128 * 128 + 128 * 14)  # 40448
 BF16_PEAK_TFLOPS = 2500.0
@@ -158,6 +204,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "fp64"])
     ap.add_argument("--collision", default="mrt", choices=["mrt", "bgk"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain2d-steps", type=int, default=5000,
+                    help="steps of the 2D FP64 plain case leg (0 skips it)")
     ap.add_argument("--surrogate-steps", type=int, default=3,
                     help="steps of the plain-vs-surrogate chemistry comparison (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -261,6 +309,8 @@ def main():
         "cpu_baseline": None,
     }
     del lb
+    if args.plain2d_steps > 0:
+        result["plain2d"] = plain2d_leg(lib, args.plain2d_steps, 0 if args.no_cpu_baseline else 1000)
     if args.surrogate_steps > 0:
         result["surrogate"] = surrogate_leg(lib, n, args.surrogate_steps)
     if not args.no_cpu_baseline:

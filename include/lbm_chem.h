@@ -96,8 +96,10 @@ typedef struct lbm_chem_mech {
 } lbm_chem_mech;
 
 /* Surrogate: x_n = (x - in_mean) * in_scale per input, y = y_n * out_scale + out_mean per
- * output, then y *= x[out_mul[o]] where out_mul[o] >= 0 (an output predicted per unit of an
- * input, e.g. the biomass change per unit of biomass); layers h1 = relu(W1 x_n + b1),
+ * output; where out_log[o] = 1 the network predicts the natural logarithm of the output's
+ * magnitude and the output is negative, y = -exp(y) (a decay that spans decades, e.g. the
+ * Arrhenius biomass conversion per step); then y *= x[out_mul[o]] where out_mul[o] >= 0 (an
+ * output predicted per unit of an input, e.g. the biomass change per unit of biomass); layers h1 = relu(W1 x_n + b1),
  * h2 = relu(W2 h1 + b2), y_n = W3 h2 + b3 with BF16 weights and layer inputs, FP32
  * accumulation.  A cell falls back to the plain integrator when any raw input leaves
  * [hull_lo, hull_hi] or when the element-balance residual of the prediction exceeds
@@ -109,6 +111,7 @@ typedef struct lbm_surrogate {
     float hull_lo[LBM_CHEM_NIN], hull_hi[LBM_CHEM_NIN];
     double residual_tol;
     int out_mul[LBM_CHEM_NOUT];            /* input index multiplying each output, -1: none */
+    int out_log[LBM_CHEM_NOUT];            /* 1: output = -exp(network output), 0: linear */
 } lbm_surrogate;
 
 typedef struct lbm_chem lbm_chem;

@@ -52,7 +52,7 @@ class Surrogate(ctypes.Structure):
     _fields_ = [
         ("params", ctypes.POINTER(_f)), ("in_mean", _f * NIN), ("in_scale", _f * NIN),
         ("out_mean", _f * NOUT), ("out_scale", _f * NOUT), ("hull_lo", _f * NIN), ("hull_hi", _f * NIN),
-        ("residual_tol", _d), ("out_mul", _i * NOUT),
+        ("residual_tol", _d), ("out_mul", _i * NOUT), ("out_log", _i * NOUT),
     ]
 
 
@@ -257,6 +257,9 @@ class Chem:
         om = np.asarray(sur.get("out_mul", [-1] * NOUT), dtype=np.int64)
         assert om.size == NOUT and np.all((om >= -1) & (om < NIN))
         s.out_mul[:] = [int(v) for v in om]
+        ol = np.asarray(sur.get("out_log", [0] * NOUT), dtype=np.int64)
+        assert ol.size == NOUT and np.all((ol == 0) | (ol == 1))
+        s.out_log[:] = [int(v) for v in ol]
         self._check(self.lib.lbm_chem_set_surrogate(self.h, ctypes.byref(s)), "lbm_chem_set_surrogate")
         self._sur = (params, s)
 

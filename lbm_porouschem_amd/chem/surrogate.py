@@ -2,8 +2,8 @@
 on the device through the library's own forward/backward kernels, and the surrogate record
 that lbm_chem_set_surrogate loads.
 
-    samples = collect(hip, shape=(96, 64, 64), steps=300, every=20)   # plain FP64 integrator
-    sur, log = train(gchem, *samples)                                  # lbm_mlp_grad + lbm_mlp_adam
+    X, Y, dt, U = collect(hip, shape=(96, 64, 64), steps=100, every=10)  # plain FP64 integrator
+    sur, log = train(gchem, X, Y, U)                                       # lbm_mlp_grad + lbm_mlp_adam
     save(path, sur)
 
 Torch appears here only as device memory for the C ABI's array arguments.
@@ -61,11 +61,18 @@ def jitter(x, rng):
     return xa
 
 
+# biomass density [kg/m^3] given to cells without biomass when their conversion rate per unit of
+# biomass is measured (the rate depends on the cell's temperature and composition only)
+B_PROBE = 1.0
+
+
 def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000, seed=0, precision="fp32",
             packing_seed=2025, jittered=True, log=print):
     """Run the coupled pyrolysis case with the plain operator attached and log (inputs, outputs)
     of the plain FP64 integrator every `every` steps, plus (jittered=True) the integrator's
-    outputs on a perturbed copy of each snapshot.  Returns (X [16, n], Y [14, n], dt)."""
+    outputs on a perturbed copy of each snapshot.  Returns (X [16, n], Y [14, n], dt, U [n]):
+    U is the biomass change per unit of biomass over the step, measured for cells without
+    biomass by a second integration with B_PROBE of biomass."""
     import torch
 
     from .. import workloads
@@ -75,28 +82,41 @@ def collect(lib, shape=(96, 64, 64), steps=300, every=20, max_samples=2_000_000,
     lb.init_simulation()
     g = Chem(lib)
     lb.attach_chemistry(g, MODE_PLAIN, rho_solid_ref=lb.chem_rho_solid_ref)
-    xs, ys = [], []
+    xs, ys, us = [], [], []
     t0 = time.perf_counter()
+
+    def integrate(x):
+        xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+        yd = torch.zeros((NOUT, x.shape[1]), dtype=torch.float64, device="cuda")
+        nd = torch.zeros(x.shape[1], dtype=torch.int32, device="cuda")
+        g.integrate(xd, yd, nd, dt=lb.dt)
+        return yd.cpu().numpy(), nd.cpu().numpy()
+
     for s in range(0, steps + 1, every):
         if s:
             lb.run(every)
         x0 = cell_inputs_from_solver(lb, rho_solid_ref=lb.chem_rho_solid_ref)
         for x in ((x0, jitter(x0, rng)) if jittered else (x0,)):
-            xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
-            yd = torch.zeros((NOUT, x.shape[1]), dtype=torch.float64, device="cuda")
-            nd = torch.zeros(x.shape[1], dtype=torch.int32, device="cuda")
-            g.integrate(xd, yd, nd, dt=lb.dt)
+            y, ns = integrate(x)
+            u = np.zeros(x.shape[1])
+            has = x[14] > 0
+            u[has] = y[12, has] / x[14, has]
+            if (~has).any():
+                xp = x[:, ~has].copy()
+                xp[14] = B_PROBE
+                u[~has] = integrate(xp)[0][12] / B_PROBE
             xs.append(x)
-            ys.append(yd.cpu().numpy())
-        ns = nd.cpu().numpy()
+            ys.append(y)
+            us.append(u)
         log(f"collect: packing {packing_seed} step {s}: {x0.shape[1]} cells, T {x0[0].min():.0f}-{x0[0].max():.0f} K, "
             f"integrator steps mean {ns.mean():.1f} max {ns.max()} ({time.perf_counter() - t0:.1f} s)")
     X = np.concatenate(xs, axis=1)
     Y = np.concatenate(ys, axis=1)
+    U = np.concatenate(us)
     if X.shape[1] > max_samples:
         idx = np.random.default_rng(seed).choice(X.shape[1], max_samples, replace=False)
-        X, Y = X[:, idx], Y[:, idx]
-    return X, Y, lb.dt
+        X, Y, U = X[:, idx], Y[:, idx], U[idx]
+    return X, Y, lb.dt, U
 
 
 def init_params(seed=0):
@@ -128,18 +148,29 @@ def normalisation(X, Y, margin=0.02):
     }
 
 
-# Output 12 (biomass change) is learned per unit of biomass (input 14): a function of the
-# temperature and composition only, exactly zero where there is no biomass.
+# Output 12 (biomass change) is learned per unit of biomass (input 14), as the logarithm of the
+# conversion per step: first-order Arrhenius decay, so a function of the temperature (and weakly
+# the composition) that spans decades, always negative, exactly zero where there is no biomass.
 OUT_MUL = [-1] * 12 + [14, -1]
+OUT_LOG = [0] * 12 + [1, 0]
+LOG_FLOOR = 1e-20
 
 
-def per_unit_targets(X, Y, out_mul=OUT_MUL):
-    """Training targets of outputs predicted per unit of an input (0 where that input is 0)."""
+def training_targets(X, Y, U=None, out_mul=OUT_MUL, out_log=OUT_LOG):
+    """Network targets before normalisation: outputs predicted per unit of an input (U, when
+    given, holds the per-unit biomass change of every sample; otherwise Y / X where X > 0 and 0
+    elsewhere), and the logarithm of the magnitude of the out_log outputs."""
     T = np.array(Y, dtype=np.float64, copy=True)
     for o, v in enumerate(out_mul):
         if v >= 0:
-            d = X[v]
-            T[o] = np.divide(Y[o], d, out=np.zeros_like(Y[o]), where=d > 0)
+            if U is not None and o == 12:
+                T[o] = U
+            else:
+                d = X[v]
+                T[o] = np.divide(Y[o], d, out=np.zeros_like(Y[o]), where=d > 0)
+    for o, f in enumerate(out_log):
+        if f:
+            T[o] = np.log(np.maximum(-T[o], LOG_FLOOR))
     return T
 
 
@@ -155,14 +186,15 @@ def normalise(X, Y, norm):
 OUT_EMPHASIS = {12: 3.0, 13: 3.0}
 
 
-def train(g, X, Y, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0.05, residual_tol=1e-3, seed=0,
+def train(g, X, Y, U=None, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0.05, residual_tol=1e-3, seed=0,
           params=None, margin=0.5, emphasis=OUT_EMPHASIS, log=print):
     """Fit the 16-128-128-14 MLP on the device with the library's kernels (lbm_mlp_grad +
-    lbm_mlp_adam); the learning rate decays by `decay` per epoch.  Returns (surrogate dict,
-    history)."""
+    lbm_mlp_adam); the learning rate decays by `decay` per epoch.  U: per-unit biomass change of
+    every sample (collect()).  Returns (surrogate dict, history; history[0] is the untrained
+    network)."""
     import torch
     rng = np.random.default_rng(seed)
-    Y = per_unit_targets(X, Y)
+    Y = training_targets(X, Y, U)
     n = X.shape[1]
     perm = rng.permutation(n)
     nh = max(int(n * holdout), 1)
@@ -181,7 +213,8 @@ def train(g, X, Y, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0.05, res
     V = torch.zeros_like(P)
     nf = Xd.shape[1]
     steps_per_epoch = max(nf // batch, 1)
-    hist = []
+    # epoch -1: the network before any update, the baseline the training is measured against
+    hist = [{"epoch": -1, "train_loss": None, "holdout_loss": holdout_loss(g, P, Xh, Yh), "seconds": 0.0}]
     t = 0
     t0 = time.perf_counter()
     rate = lr
@@ -205,6 +238,7 @@ def train(g, X, Y, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0.05, res
     sur["params"] = P.cpu().numpy()
     sur["residual_tol"] = residual_tol
     sur["out_mul"] = np.array(OUT_MUL, dtype=np.int32)
+    sur["out_log"] = np.array(OUT_LOG, dtype=np.int32)
     return sur, hist
 
 

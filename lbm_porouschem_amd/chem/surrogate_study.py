@@ -166,13 +166,14 @@ def main():
                            packing_seed=sd, log=log) for sd in seeds]
         X = np.concatenate([p[0] for p in parts], axis=1)
         Y = np.concatenate([p[1] for p in parts], axis=1)
+        U = np.concatenate([p[3] for p in parts])
         if X.shape[1] > args.samples:
             idx = np.random.default_rng(0).choice(X.shape[1], args.samples, replace=False)
-            X, Y = X[:, idx], Y[:, idx]
+            X, Y, U = X[:, idx], Y[:, idx], U[idx]
         res["collect"] = {"shape": list(shape), "packing_seeds": seeds, "steps": args.train_steps,
                           "every": args.every, "samples": int(X.shape[1]), "seconds": time.perf_counter() - t0}
         g = Chem(lib)
-        sur, hist = S.train(g, X, Y, epochs=args.epochs, log=log)
+        sur, hist = S.train(g, X, Y, U, epochs=args.epochs, log=log)
         res["train"] = {"epochs": args.epochs, "history": hist}
         S.save(os.path.join(args.out, "surrogate.npz"), sur)
     ev = tuple(int(v) for v in args.eval_shape.split(","))

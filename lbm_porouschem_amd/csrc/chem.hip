@@ -11,10 +11,12 @@
 //                     Rosenbrock (ROS2) integration over dt in FP64 one attempt at a time
 //                     (finite-difference Jacobian reused across accepted steps, pivoted LU),
 //                     hand the result to the source policy, take the next cell.
-//   k_surrogate<Src>  one wavefront per 16 cells: inputs staged through LDS, three MLP
-//                     layers as v_mfma_f32_16x16x16_bf16 tiles with the weights in LDS
-//                     (fragment order, conflict-free 8-byte reads), hull and element-
-//                     balance checks, fallback cells appended to a list.
+//   k_surrogate<Src>  64 cells per wavefront: every lane gathers one cell's inputs into an
+//                     LDS row, the MLP runs as four 16-cell tiles (layer 1 on
+//                     v_mfma_f32_16x16x16_bf16, the 128-wide layers on gfx950's
+//                     v_mfma_f32_16x16x32_bf16, weights in LDS in fragment order), then
+//                     every lane checks its cell (hull, element balance); fallback cells
+//                     are appended to a list.
 //   k_plain<Src> over a list: the plain integrator on the surrogate's fallback cells.
 //   k_train / k_wgrad / k_adam   FP32 training: per-sample forward + backward with
 //                     BF16-rounded operands, per-parameter gradient reduction in FP64,
@@ -47,7 +49,8 @@ constexpr int OFF_W1 = 0, OFF_B1 = HID * NI, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF
 constexpr double R_GAS = 8.314462618;
 constexpr double P_ATM = 101325.0;
 constexpr int PLAIN_BLOCK = 128;
-constexpr int SUR_BLOCK = 256;  // 4 wavefronts, 16 cells each per tile
+constexpr int SUR_WAVES = 8;
+constexpr int SUR_BLOCK = 64 * SUR_WAVES;  // surrogate: 64 cells per wavefront per iteration
 
 // ---------------------------------------------------------------------------
 // FP64 operator (same algorithm and operation order as the CPU checker)
@@ -528,21 +531,31 @@ __global__ void k_zero_u32(unsigned* p, int n) {
 // MLP on the matrix cores
 // ---------------------------------------------------------------------------
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// weight tiles in fragment order: tile t, lane l, element e at w[(t * 64 + l) * 4 + e]
-// holds A[l % 16][4 * (l / 16) + e] of that tile (16 x 16, rows = neurons)
-constexpr int TILE_W1 = 0;             // 8 row tiles x 1 k tile
-constexpr int TILE_W2 = 8;             // 8 row tiles x 8 k tiles (t = 8 + mt * 8 + kt)
-constexpr int TILE_W3 = 72;            // 1 row tile (outputs 14, 15 zero) x 8 k tiles
-constexpr int NTILE = 80;
+// Weights in fragment order (rows = neurons; the layers are computed transposed, neurons x
+// cells, 16 cells per tile):
+//   W1 (128 x 16): 8 tiles of v_mfma_f32_16x16x16_bf16, element (t, lane l, e) at
+//       [t * 256 + l * 4 + e] = W1[16 t + l % 16][4 (l / 16) + e];
+//   W2 (128 x 128): 8 row tiles x 4 k tiles of v_mfma_f32_16x16x32_bf16 (gfx950), tile
+//       u = 4 mt + kt, element (u, l, j) at [W2_BASE + u * 512 + l * 8 + j] = W2[16 mt + l % 16][kperm];
+//   W3 (14 x 128, rows 14 and 15 zero): 4 k tiles at [W3_BASE + kt * 512 + l * 8 + j].
+// kperm(kt, l, j) = 16 (2 kt + j / 4) + 4 (l / 16) + j % 4 orders the 32 k values of a tile so
+// that lane l's B operand is its own accumulator values of the previous layer's row tiles 2 kt
+// and 2 kt + 1 (4 each): the layers chain in registers, with no lane exchange.
+constexpr int W2_BASE = 8 * 256;
+constexpr int W3_BASE = W2_BASE + 32 * 512;
+constexpr int W_ELEMS = W3_BASE + 4 * 512;  // 20480 BF16 values
 
 struct SurDev {
-    unsigned short w[NTILE * 256];
+    unsigned short w[W_ELEMS];
     float b1[HID], b2[HID], b3[16];
     float in_mean[16], in_scale[16], out_mean[16], out_scale[16], lo[16], hi[16];
     double residual_tol;
     int out_mul[16];
+    int out_log[16];
 };
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -560,24 +573,26 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     r[3] = (short)bf16_bits(d);
     return r;
 }
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+    const s16x8 v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ int kperm(int kt, int l, int j) { return 16 * (2 * kt + (j >> 2)) + 4 * (l >> 4) + (j & 3); }
 
 // FP32 parameters -> fragment-ordered BF16 tiles and biases of a SurDev
 __global__ void k_pack(const float* __restrict__ P, SurDev* __restrict__ S) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < NTILE * 256) {
+    if (g < W2_BASE) {
         const int t = g >> 8, l = (g >> 2) & 63, e = g & 3;
-        const int r = l & 15, k = 4 * (l >> 4) + e;
-        float v = 0.0f;
-        if (t < TILE_W2) {
-            v = P[OFF_W1 + (t * 16 + r) * NI + k];
-        } else if (t < TILE_W3) {
-            const int mt = (t - TILE_W2) >> 3, kt = (t - TILE_W2) & 7;
-            v = P[OFF_W2 + (mt * 16 + r) * HID + kt * 16 + k];
-        } else {
-            const int kt = t - TILE_W3;
-            v = r < NO ? P[OFF_W3 + r * HID + kt * 16 + k] : 0.0f;
-        }
-        S->w[g] = bf16_bits(v);
+        S->w[g] = bf16_bits(P[OFF_W1 + (t * 16 + (l & 15)) * NI + 4 * (l >> 4) + e]);
+    } else if (g < W3_BASE) {
+        const int u = (g - W2_BASE) >> 9, l = ((g - W2_BASE) >> 3) & 63, j = g & 7;
+        const int mt = u >> 2, kt = u & 3;
+        S->w[g] = bf16_bits(P[OFF_W2 + (mt * 16 + (l & 15)) * HID + kperm(kt, l, j)]);
+    } else if (g < W_ELEMS) {
+        const int kt = (g - W3_BASE) >> 9, l = ((g - W3_BASE) >> 3) & 63, j = g & 7;
+        const int r = l & 15;
+        S->w[g] = bf16_bits(r < NO ? P[OFF_W3 + r * HID + kperm(kt, l, j)] : 0.0f);
     }
     if (g < HID) {
         S->b1[g] = P[OFF_B1 + g];
@@ -586,25 +601,31 @@ __global__ void k_pack(const float* __restrict__ P, SurDev* __restrict__ S) {
     if (g < 16) S->b3[g] = g < NO ? P[OFF_B3 + g] : 0.0f;
 }
 
-__device__ __forceinline__ bf16x4 frag(const unsigned short* wl, int t) {
+__device__ __forceinline__ bf16x4 frag4(const unsigned short* wl, int t) {
     return *(const bf16x4*)(wl + ((t * 64 + (int)(threadIdx.x & 63)) << 2));
 }
+__device__ __forceinline__ bf16x8 frag8(const unsigned short* wl, int base, int t) {
+    return __builtin_bit_cast(bf16x8, *(const s16x8*)(wl + base + ((t * 64 + (int)(threadIdx.x & 63)) << 3)));
+}
 
-// One wavefront evaluates the MLP for 16 cells: xin[cell][16] (normalised, LDS) ->
-// yout[cell][16] (normalised outputs, LDS).  Layers are computed transposed
-// (neurons x cells), so each layer's accumulator tile is directly the next layer's
-// B fragment: no LDS round trip between layers.
-__device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __restrict__ bl, const float* xin,
-                         float* yout) {
+// Row stride (floats) of the per-wavefront cell rows in LDS: odd, so that lane l writing
+// row l touches bank (17 l + v) % 64, a different bank per lane.
+constexpr int XS = 17;
+
+// One wavefront evaluates the MLP for 16 cells: rows[cell][0..15] (normalised inputs, LDS) ->
+// rows[cell][0..15] (normalised outputs; 14 and 15 are zero rows of W3).  Lane l reads and
+// writes only its own four positions (cell l % 16, values 4 (l / 16) .. + 3), so the inputs
+// can be overwritten in place.
+__device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __restrict__ bl, float* rows) {
     const int l = threadIdx.x & 63, q = l >> 4, cc = l & 15;
-    const float* xr = xin + cc * 16 + 4 * q;
-    const bf16x4 xb = pack4(xr[0], xr[1], xr[2], xr[3]);
+    float* r = rows + cc * XS + 4 * q;
+    const bf16x4 xb = pack4(r[0], r[1], r[2], r[3]);
     bf16x4 h1[8], h2[8];
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
         const float* b = bl + mt * 16 + 4 * q;
         f32x4 acc = {b[0], b[1], b[2], b[3]};
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(wl, TILE_W1 + mt), xb, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag4(wl, mt), xb, acc, 0, 0, 0);
         h1[mt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
     }
 #pragma unroll
@@ -612,36 +633,38 @@ __device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __r
         const float* b = bl + HID + mt * 16 + 4 * q;
         f32x4 acc = {b[0], b[1], b[2], b[3]};
 #pragma unroll
-        for (int kt = 0; kt < 8; ++kt)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(wl, TILE_W2 + mt * 8 + kt), h1[kt], acc, 0, 0, 0);
+        for (int kt = 0; kt < 4; ++kt)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag8(wl, W2_BASE, mt * 4 + kt), cat8(h1[2 * kt], h1[2 * kt + 1]),
+                                                          acc, 0, 0, 0);
         h2[mt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
     }
     const float* b = bl + 2 * HID + 4 * q;
     f32x4 acc = {b[0], b[1], b[2], b[3]};
 #pragma unroll
-    for (int kt = 0; kt < 8; ++kt)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(wl, TILE_W3 + kt), h2[kt], acc, 0, 0, 0);
-    float* yr = yout + cc * 16 + 4 * q;
-    yr[0] = acc[0];
-    yr[1] = acc[1];
-    yr[2] = acc[2];
-    yr[3] = acc[3];
+    for (int kt = 0; kt < 4; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag8(wl, W3_BASE, kt), cat8(h2[2 * kt], h2[2 * kt + 1]), acc, 0, 0,
+                                                      0);
+    r[0] = acc[0];
+    r[1] = acc[1];
+    r[2] = acc[2];
+    r[3] = acc[3];
 }
 
-// Surrogate over n cells.  CHECK = 0: normalised in/out (lbm_mlp_forward).
+// Surrogate over n cells, SUR_BLOCK cells per block iteration (64 per wavefront: every lane
+// gathers and checks one cell; the wavefront's MLP runs as four 16-cell tiles).
+// CHECK = 0: normalised in/out (lbm_mlp_forward).
 template <class Src, int CHECK>
 __global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restrict__ S,
                                                          const lbm_chem_mech* __restrict__ mech, Src src, long long n,
                                                          unsigned* list, unsigned* count,
                                                          unsigned long long* stats) {
-    __shared__ __attribute__((aligned(16))) unsigned short wl[NTILE * 256];
+    __shared__ __attribute__((aligned(16))) unsigned short wl[W_ELEMS];
     __shared__ __attribute__((aligned(16))) float bl[2 * HID + 16];
-    __shared__ __attribute__((aligned(16))) float xs[4][256];
-    __shared__ __attribute__((aligned(16))) float ys[4][256];
+    __shared__ float rows[SUR_WAVES][64 * XS];
     {
         const uint4* gw = (const uint4*)S->w;
         uint4* lw = (uint4*)wl;
-        for (int i = threadIdx.x; i < NTILE * 256 * 2 / 16; i += SUR_BLOCK) lw[i] = gw[i];
+        for (int i = threadIdx.x; i < W_ELEMS * 2 / 16; i += SUR_BLOCK) lw[i] = gw[i];
         for (int i = threadIdx.x; i < HID; i += SUR_BLOCK) {
             bl[i] = S->b1[i];
             bl[HID + i] = S->b2[i];
@@ -650,33 +673,38 @@ __global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restric
     }
     __syncthreads();
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const long long ntile = (n + 63) / 64;
+    float* buf = rows[wv];
+    const long long ntile = (n + SUR_BLOCK - 1) / SUR_BLOCK;
     for (long long t = blockIdx.x; t < ntile; t += gridDim.x) {
-        const long long cell = t * 64 + wv * 16 + (l & 15);
+        const long long cell = t * SUR_BLOCK + wv * 64 + l;
         double x[NI];
         bool act = false;
-        if (l < 16) {
-            if constexpr (CHECK) {
-                if (cell < n) act = src.load(cell, x);
+        if constexpr (CHECK) {
+            if (cell < n) act = src.load(cell, x);
 #pragma unroll
-                for (int v = 0; v < NI; ++v)
-                    xs[wv][l * 16 + v] = act ? ((float)x[v] - S->in_mean[v]) * S->in_scale[v] : 0.0f;
-            } else {
-                act = cell < n;
+            for (int v = 0; v < NI; ++v) buf[l * XS + v] = act ? ((float)x[v] - S->in_mean[v]) * S->in_scale[v] : 0.0f;
+        } else {
+            act = cell < n;
 #pragma unroll
-                for (int v = 0; v < NI; ++v) xs[wv][l * 16 + v] = act ? src.in[(long long)v * src.n + cell] : 0.0f;
-            }
+            for (int v = 0; v < NI; ++v) buf[l * XS + v] = act ? src.in[(long long)v * src.n + cell] : 0.0f;
         }
         __syncthreads();
-        mlp_tile(wl, bl, xs[wv], ys[wv]);
+        // the weight fragments are re-read from LDS for every tile: the empty asm with a memory
+        // clobber keeps the compiler from hoisting them into (160) registers across the loop
+#pragma unroll 1
+        for (int s = 0; s < 4; ++s) {
+            asm volatile("" ::: "memory");
+            mlp_tile(wl, bl, buf + s * 16 * XS);
+        }
         __syncthreads();
         bool hull = false, res = false;
-        if (l < 16 && act) {
+        if (act) {
             if constexpr (CHECK) {
                 double y[NO];
 #pragma unroll
                 for (int o = 0; o < NO; ++o) {
-                    y[o] = (double)(ys[wv][l * 16 + o] * S->out_scale[o] + S->out_mean[o]);
+                    y[o] = (double)(buf[l * XS + o] * S->out_scale[o] + S->out_mean[o]);
+                    if (S->out_log[o]) y[o] = -exp(y[o]);
                     const int mv = S->out_mul[o];
                     if (mv >= 0) y[o] *= x[mv];
                 }
@@ -687,7 +715,7 @@ __global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restric
                 src.predicted(cell, x, y, hull ? 1 : (res ? 2 : 0));
             } else {
 #pragma unroll
-                for (int o = 0; o < NO; ++o) src.out[(long long)o * src.n + cell] = ys[wv][l * 16 + o];
+                for (int o = 0; o < NO; ++o) src.out[(long long)o * src.n + cell] = buf[l * XS + o];
             }
         }
         if constexpr (CHECK) {
@@ -896,9 +924,10 @@ int after_default_stream(lbm_chem* c) {
     return LBM_OK;
 }
 
+// persistent surrogate pass: two blocks per CU fit in LDS (77 KB each)
 unsigned grid_tiles(const lbm_chem* c, long long n) {
-    const long long tiles = (n + 63) / 64;
-    const long long cap = (long long)c->n_cu * 8;
+    const long long tiles = (n + SUR_BLOCK - 1) / SUR_BLOCK;
+    const long long cap = (long long)c->n_cu * 2;
     return (unsigned)(tiles < cap ? (tiles > 0 ? tiles : 1) : cap);
 }
 
@@ -1023,10 +1052,11 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
     }
     if (!s->params) return cerr(c, LBM_ERR_INVALID, "surrogate without parameters");
     for (int o = 0; o < NO; ++o)
-        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI) return cerr(c, LBM_ERR_INVALID, "out_mul[%d] = %d", o, s->out_mul[o]);
+        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI || (s->out_log[o] != 0 && s->out_log[o] != 1))
+            return cerr(c, LBM_ERR_INVALID, "out_mul[%d] = %d, out_log[%d] = %d", o, s->out_mul[o], o, s->out_log[o]);
     CHEM_TRY(hipSetDevice(c->device));
     CHEM_TRY(hipMemcpyAsync(c->d_params, s->params, NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_pack, dim3((NTILE * 256 + 255) / 256), dim3(256), 0, c->stream, c->d_params, c->d_sur);
+    hipLaunchKernelGGL(k_pack, dim3((W_ELEMS + 255) / 256), dim3(256), 0, c->stream, c->d_params, c->d_sur);
     CHEM_TRY(hipGetLastError());
     // normalisation, hull and tolerance
     SurDev host_tail;
@@ -1040,7 +1070,10 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
         host_tail.out_scale[i] = i < NO ? s->out_scale[i] : 0.0f;
     }
     host_tail.residual_tol = s->residual_tol;
-    for (int o = 0; o < 16; ++o) host_tail.out_mul[o] = o < NO ? s->out_mul[o] : -1;
+    for (int o = 0; o < 16; ++o) {
+        host_tail.out_mul[o] = o < NO ? s->out_mul[o] : -1;
+        host_tail.out_log[o] = o < NO ? s->out_log[o] : 0;
+    }
     const size_t off = offsetof(SurDev, in_mean);
     CHEM_TRY(hipMemcpyAsync((char*)c->d_sur + off, (const char*)&host_tail + off, sizeof(SurDev) - off,
                             hipMemcpyHostToDevice, c->stream));
@@ -1077,7 +1110,7 @@ int lbm_mlp_forward(lbm_chem* c, const float* params, const float* xn, float* yn
     if (n == 0) return LBM_OK;
     CHEM_TRY(hipSetDevice(c->device));
     if (int rc0 = after_default_stream(c)) return rc0;
-    hipLaunchKernelGGL(k_pack, dim3((NTILE * 256 + 255) / 256), dim3(256), 0, c->stream, params, c->d_tmp);
+    hipLaunchKernelGGL(k_pack, dim3((W_ELEMS + 255) / 256), dim3(256), 0, c->stream, params, c->d_tmp);
     SrcNorm src{xn, yn, n};
     hipLaunchKernelGGL((k_surrogate<SrcNorm, 0>), dim3(grid_tiles(c, n)), dim3(SUR_BLOCK), 0, c->stream, c->d_tmp,
                        c->d_mech, src, n, (unsigned*)nullptr, (unsigned*)nullptr, (unsigned long long*)nullptr);

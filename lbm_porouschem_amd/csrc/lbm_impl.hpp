@@ -59,11 +59,29 @@ __global__ __launch_bounds__(BLOCK) void k_init(const DevParams<Real>* __restric
 #ifndef LBM_WAVES_PER_EU
 #define LBM_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
+// Block order.  Workgroups are dispatched round-robin over the 8 XCDs of the chip (block b
+// runs on XCD b % 8).  With LBM_XCD_REMAP=1, xcd_block() renumbers them so that each XCD walks
+// one contiguous eighth of the lattice (L2 locality per XCD).  Measured on the 256^3 coupled
+// step, that is 13 % slower (2.20 vs 1.94 ms per bulk launch): in natural order all XCDs stream
+// through one contiguous window of each of the 61 population arrays at a time, and for a pure
+// streaming kernel HBM page locality matters more than sharing lines within one L2 (the
+// z-shifted accesses wrap inside a row, which one workgroup owns).  Natural order is the default.
+constexpr unsigned NXCD = 8;
+#ifndef LBM_XCD_REMAP
+#define LBM_XCD_REMAP 0
+#endif
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    if (!LBM_XCD_REMAP) return b;
+    const unsigned x = b % NXCD, loc = b / NXCD, q = G / NXCD, r = G % NXCD;
+    return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+}
+
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
 __global__ __launch_bounds__(BLOCK) LBM_WAVES_PER_EU(M::exact_species ? LBM_BULK_WAVES : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
-    long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    long long c = (long long)xcd_block() * BLOCK + threadIdx.x;
     if (c >= P.N) return;
     bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }

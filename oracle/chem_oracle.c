@@ -482,7 +482,8 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
     }
     if (!s->params) return LBM_ERR_INVALID;
     for (int o = 0; o < NO; ++o)
-        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI) return LBM_ERR_INVALID;
+        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI || (s->out_log[o] != 0 && s->out_log[o] != 1))
+            return LBM_ERR_INVALID;
     memcpy(c->params, s->params, sizeof(c->params));
     c->sur = *s;
     c->sur.params = c->params;
@@ -502,6 +503,7 @@ int chem_oracle_surrogate_cell(const lbm_chem* c, const double* in, double dt, i
     mlp_fwd(c->params, xn, &t);
     for (int o = 0; o < NO; ++o) {
         out[o] = (double)((float)t.y[o] * s->out_scale[o] + s->out_mean[o]);
+        if (s->out_log[o]) out[o] = -exp(out[o]);
         if (s->out_mul[o] >= 0) out[o] *= in[s->out_mul[o]];
     }
     if (!flag && lbm_chem_residual(&c->mech, in, out) > s->residual_tol) flag = 2;

@@ -46,8 +46,9 @@ def test_trained_surrogate_gains_over_plain_and_untrained(hip):
              for sd in (2025, 2027)]
     X = np.concatenate([p[0] for p in parts], axis=1)
     Y = np.concatenate([p[1] for p in parts], axis=1)
+    U = np.concatenate([p[3] for p in parts])
     g = chem.Chem(hip)
-    sur, hist = S.train(g, X, Y, epochs=25, batch=4096, log=lambda m: None)
+    sur, hist = S.train(g, X, Y, U, epochs=25, batch=4096, log=lambda m: None)
     assert hist[-1]["holdout_loss"] < 0.2 * hist[0]["holdout_loss"]
 
     tf_p, ts_p, conv_p, st_p = run(hip, chem.MODE_PLAIN, None, 2026)
