@@ -143,7 +143,7 @@ def surrogate_leg(lib, n, steps):
     the committed MLP surrogate (per-cell fallback on).  Returns per-step times, the speedup,
     the fields' difference after those steps and the surrogate kernel's matrix-core rate."""
     import numpy as np
-    from lbm_porouschem_amd import chem
+    from lbm_porouschem_amd import _abi, chem, workloads
     from lbm_porouschem_amd.chem import surrogate as S
     if not os.path.exists(SURROGATE_FILE):
         return None

@@ -95,11 +95,20 @@ typedef struct lbm_chem_mech {
                                               ROS2 keeps its order with an inexact Jacobian. */
 } lbm_chem_mech;
 
-/* Surrogate: x_n = (x - in_mean) * in_scale per input, y = y_n * out_scale + out_mean per
- * output; where out_log[o] = 1 the network predicts the natural logarithm of the output's
- * magnitude and the output is negative, y = -exp(y) (a decay that spans decades, e.g. the
- * Arrhenius biomass conversion per step); then y *= x[out_mul[o]] where out_mul[o] >= 0 (an
- * output predicted per unit of an input, e.g. the biomass change per unit of biomass); layers h1 = relu(W1 x_n + b1),
+/* Surrogate: x_n = (x - in_mean) * in_scale per input, z = y_n * out_scale + out_mean per
+ * output, then by out_kind[o]:
+ *   0  y = z;
+ *   1  y = -exp(z): the network predicts the logarithm of a negative output's magnitude (a
+ *      decay spanning decades, e.g. the Arrhenius biomass conversion per step);
+ *   2  y = out_lin[o] * sinh(z): linear for |y| << out_lin, logarithmic beyond (changes that
+ *      are tiny in most cells and large in a few);
+ *   3  gas species only: y = exp(z) - rho_k, the network predicts the logarithm of the species'
+ *      partial density at the end of the step (species that relax to a quasi-steady value
+ *      within a step, e.g. the H, O, OH radicals); rho_k = rho_g Y_k at the start, with the gas
+ *      density rho_g = p / (R T sum_k Y_k / M_k);
+ * then y *= x[out_mul[o]] where out_mul[o] >= 0 (an output predicted per unit of an input, e.g.
+ * the biomass change per unit of biomass), and a gas species output is raised to -rho_k where it
+ * is below (no species loses more than it has); layers h1 = relu(W1 x_n + b1),
  * h2 = relu(W2 h1 + b2), y_n = W3 h2 + b3 with BF16 weights and layer inputs, FP32
  * accumulation.  A cell falls back to the plain integrator when any raw input leaves
  * [hull_lo, hull_hi] or when the element-balance residual of the prediction exceeds
@@ -111,7 +120,8 @@ typedef struct lbm_surrogate {
     float hull_lo[LBM_CHEM_NIN], hull_hi[LBM_CHEM_NIN];
     double residual_tol;
     int out_mul[LBM_CHEM_NOUT];            /* input index multiplying each output, -1: none */
-    int out_log[LBM_CHEM_NOUT];            /* 1: output = -exp(network output), 0: linear */
+    int out_kind[LBM_CHEM_NOUT];           /* output transform 0..3 (above) */
+    float out_lin[LBM_CHEM_NOUT];          /* linear scale of out_kind 2 (> 0) */
 } lbm_surrogate;
 
 typedef struct lbm_chem lbm_chem;

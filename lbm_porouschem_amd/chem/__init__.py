@@ -52,7 +52,8 @@ class Surrogate(ctypes.Structure):
     _fields_ = [
         ("params", ctypes.POINTER(_f)), ("in_mean", _f * NIN), ("in_scale", _f * NIN),
         ("out_mean", _f * NOUT), ("out_scale", _f * NOUT), ("hull_lo", _f * NIN), ("hull_hi", _f * NIN),
-        ("residual_tol", _d), ("out_mul", _i * NOUT), ("out_log", _i * NOUT),
+        ("residual_tol", _d), ("out_mul", _i * NOUT), ("out_kind", _i * NOUT),
+        ("out_lin", _f * NOUT),
     ]
 
 
@@ -257,9 +258,12 @@ class Chem:
         om = np.asarray(sur.get("out_mul", [-1] * NOUT), dtype=np.int64)
         assert om.size == NOUT and np.all((om >= -1) & (om < NIN))
         s.out_mul[:] = [int(v) for v in om]
-        ol = np.asarray(sur.get("out_log", [0] * NOUT), dtype=np.int64)
-        assert ol.size == NOUT and np.all((ol == 0) | (ol == 1))
-        s.out_log[:] = [int(v) for v in ol]
+        ok = np.asarray(sur.get("out_kind", [0] * NOUT), dtype=np.int64)
+        assert ok.size == NOUT and np.all((ok >= 0) & (ok <= 3)) and np.all(ok[NSP:] != 3)
+        s.out_kind[:] = [int(v) for v in ok]
+        lin = np.asarray(sur.get("out_lin", [1.0] * NOUT), dtype=np.float32)
+        assert lin.size == NOUT and np.all(lin[ok == 2] > 0)
+        s.out_lin[:] = lin.tolist()
         self._check(self.lib.lbm_chem_set_surrogate(self.h, ctypes.byref(s)), "lbm_chem_set_surrogate")
         self._sur = (params, s)
 

@@ -148,18 +148,49 @@ def normalisation(X, Y, margin=0.02):
     }
 
 
-# Output 12 (biomass change) is learned per unit of biomass (input 14), as the logarithm of the
-# conversion per step: first-order Arrhenius decay, so a function of the temperature (and weakly
-# the composition) that spans decades, always negative, exactly zero where there is no biomass.
+# Output transforms (lbm_surrogate.out_kind):
+#  - output 12 (biomass change) is learned per unit of biomass (input 14), as the logarithm of the
+#    conversion per step (kind 1): first-order Arrhenius decay, a function of the temperature (and
+#    weakly the composition) that spans decades, always negative, exactly zero without biomass;
+#  - gas species that relax to a quasi-steady value within one step (their change cancels most of
+#    what they held: median |d rho_k| / rho_k > FAST_RESET over the data) are learned as the
+#    logarithm of their partial density at the end of the step (kind 3): learning the change would
+#    ask the network to cancel the current value exactly, and its error then drives the species
+#    negative or lets it drift by orders of magnitude;
+#  - the other gas species changes and the temperature change are learned as asinh(y / s) (kind 2),
+#    s = the median magnitude of the output over the data: linear in the many quiet cells, whose
+#    tiny changes must stay tiny, logarithmic in the few reacting ones.
 OUT_MUL = [-1] * 12 + [14, -1]
-OUT_LOG = [0] * 12 + [1, 0]
+OUT_KIND = [2] * 12 + [1, 2]   # before the choice of kind-3 species (choose_kinds)
+FAST_RESET = 0.5
 LOG_FLOOR = 1e-20
+RHO_FLOOR = 1e-30
+LIN_FLOOR = 1e-30
 
 
-def training_targets(X, Y, U=None, out_mul=OUT_MUL, out_log=OUT_LOG):
-    """Network targets before normalisation: outputs predicted per unit of an input (U, when
-    given, holds the per-unit biomass change of every sample; otherwise Y / X where X > 0 and 0
-    elsewhere), and the logarithm of the magnitude of the out_log outputs."""
+def gas_density(X):
+    """rho_g = p / (R T sum_k Y_k / M_k) of each sample [kg/m^3] (the decode's formula)."""
+    from . import R_GAS, load_mechanism_table
+    M = np.array([sp["M"] for sp in load_mechanism_table()["species"]])
+    sym = (X[2:14] / M[:, None]).sum(axis=0)
+    ok = (sym > 0) & (X[0] > 0)
+    return np.where(ok, X[1] / (R_GAS * np.where(ok, X[0], 1.0) * np.where(ok, sym, 1.0)), 0.0)
+
+
+def choose_kinds(X, Y, base=OUT_KIND):
+    """out_kind per output: kind 3 for the gas species that reset within a step."""
+    kinds = list(base)
+    have = gas_density(X)[None, :] * X[2:14]
+    for k in range(12):
+        live = have[k] > 0
+        if live.sum() and np.median(np.abs(Y[k, live]) / have[k, live]) > FAST_RESET:
+            kinds[k] = 3
+    return kinds
+
+
+def per_unit(X, Y, U=None, out_mul=OUT_MUL):
+    """Outputs predicted per unit of an input (U, when given, holds the per-unit biomass change of
+    every sample; otherwise Y / X where X > 0 and 0 elsewhere)."""
     T = np.array(Y, dtype=np.float64, copy=True)
     for o, v in enumerate(out_mul):
         if v >= 0:
@@ -168,10 +199,37 @@ def training_targets(X, Y, U=None, out_mul=OUT_MUL, out_log=OUT_LOG):
             else:
                 d = X[v]
                 T[o] = np.divide(Y[o], d, out=np.zeros_like(Y[o]), where=d > 0)
-    for o, f in enumerate(out_log):
-        if f:
-            T[o] = np.log(np.maximum(-T[o], LOG_FLOOR))
     return T
+
+
+def linear_scales(T, out_kind=OUT_KIND):
+    """out_lin of the kind-2 outputs: the median magnitude of the nonzero values (1 elsewhere)."""
+    lin = np.ones(len(out_kind), dtype=np.float64)
+    for o, k in enumerate(out_kind):
+        if k == 2:
+            a = np.abs(T[o])
+            a = a[a > 0]
+            lin[o] = max(float(np.median(a)), LIN_FLOOR) if a.size else 1.0
+    return lin.astype(np.float32)
+
+
+def training_targets(X, Y, U=None, out_mul=OUT_MUL, out_kind=None, out_lin=None):
+    """Network targets before normalisation (z of lbm_surrogate): per-unit outputs, then the
+    inverse of each output's transform.  Returns (targets, out_kind, out_lin)."""
+    if out_kind is None:
+        out_kind = choose_kinds(X, Y)
+    T = per_unit(X, Y, U, out_mul)
+    if out_lin is None:
+        out_lin = linear_scales(T, out_kind)
+    rho_g = gas_density(X)
+    for o, k in enumerate(out_kind):
+        if k == 1:
+            T[o] = np.log(np.maximum(-T[o], LOG_FLOOR))
+        elif k == 2:
+            T[o] = np.arcsinh(T[o] / np.float64(out_lin[o]))
+        elif k == 3:
+            T[o] = np.log(np.maximum(rho_g * X[2 + o] + T[o], RHO_FLOOR))
+    return T, list(out_kind), out_lin
 
 
 def normalise(X, Y, norm):
@@ -194,7 +252,7 @@ def train(g, X, Y, U=None, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0
     network)."""
     import torch
     rng = np.random.default_rng(seed)
-    Y = training_targets(X, Y, U)
+    Y, out_kind, out_lin = training_targets(X, Y, U)
     n = X.shape[1]
     perm = rng.permutation(n)
     nh = max(int(n * holdout), 1)
@@ -238,7 +296,8 @@ def train(g, X, Y, U=None, epochs=40, batch=8192, lr=1e-3, decay=0.92, holdout=0
     sur["params"] = P.cpu().numpy()
     sur["residual_tol"] = residual_tol
     sur["out_mul"] = np.array(OUT_MUL, dtype=np.int32)
-    sur["out_log"] = np.array(OUT_LOG, dtype=np.int32)
+    sur["out_kind"] = np.array(out_kind, dtype=np.int32)
+    sur["out_lin"] = out_lin
     return sur, hist
 
 

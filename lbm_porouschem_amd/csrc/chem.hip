@@ -555,7 +555,8 @@ struct SurDev {
     float in_mean[16], in_scale[16], out_mean[16], out_scale[16], lo[16], hi[16];
     double residual_tol;
     int out_mul[16];
-    int out_log[16];
+    int out_kind[16];
+    float out_lin[16];
 };
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -606,6 +607,24 @@ __device__ __forceinline__ bf16x4 frag4(const unsigned short* wl, int t) {
 }
 __device__ __forceinline__ bf16x8 frag8(const unsigned short* wl, int base, int t) {
     return __builtin_bit_cast(bf16x8, *(const s16x8*)(wl + base + ((t * 64 + (int)(threadIdx.x & 63)) << 3)));
+}
+
+// Physical outputs of one cell from the network's normalised outputs yn (lbm_surrogate: output
+// transform by out_kind, per-unit scaling by out_mul, no gas species below zero).
+__device__ void decode(const SurDev& S, const lbm_chem_mech& m, const double* x, const float* yn, double* y) {
+    double sym = 0.0;
+    for (int k = 0; k < NS; ++k) sym += x[2 + k] / m.molemass[k];
+    const double rho_g = sym > 0.0 && x[0] > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+        const double z = (double)(yn[o] * S.out_scale[o] + S.out_mean[o]);
+        const double have = o < NS ? rho_g * x[2 + o] : 0.0;
+        const int kd = S.out_kind[o];
+        double v = kd == 1 ? -exp(z) : (kd == 2 ? (double)S.out_lin[o] * sinh(z) : (kd == 3 ? exp(z) - have : z));
+        if (S.out_mul[o] >= 0) v *= x[S.out_mul[o]];
+        if (o < NS && v < -have) v = -have;
+        y[o] = v;
+    }
 }
 
 // Row stride (floats) of the per-wavefront cell rows in LDS: odd, so that lane l writing
@@ -701,13 +720,7 @@ __global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restric
         if (act) {
             if constexpr (CHECK) {
                 double y[NO];
-#pragma unroll
-                for (int o = 0; o < NO; ++o) {
-                    y[o] = (double)(buf[l * XS + o] * S->out_scale[o] + S->out_mean[o]);
-                    if (S->out_log[o]) y[o] = -exp(y[o]);
-                    const int mv = S->out_mul[o];
-                    if (mv >= 0) y[o] *= x[mv];
-                }
+                decode(*S, *mech, x, buf + l * XS, y);
 #pragma unroll
                 for (int v = 0; v < NI; ++v)
                     if (x[v] < (double)S->lo[v] || x[v] > (double)S->hi[v]) hull = true;
@@ -1052,8 +1065,10 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
     }
     if (!s->params) return cerr(c, LBM_ERR_INVALID, "surrogate without parameters");
     for (int o = 0; o < NO; ++o)
-        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI || (s->out_log[o] != 0 && s->out_log[o] != 1))
-            return cerr(c, LBM_ERR_INVALID, "out_mul[%d] = %d, out_log[%d] = %d", o, s->out_mul[o], o, s->out_log[o]);
+        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI || s->out_kind[o] < 0 || s->out_kind[o] > 3 ||
+            (s->out_kind[o] == 2 && !(s->out_lin[o] > 0.0f)) || (s->out_kind[o] == 3 && o >= NS))
+            return cerr(c, LBM_ERR_INVALID, "output %d: out_mul %d, out_kind %d, out_lin %g", o, s->out_mul[o],
+                        s->out_kind[o], (double)s->out_lin[o]);
     CHEM_TRY(hipSetDevice(c->device));
     CHEM_TRY(hipMemcpyAsync(c->d_params, s->params, NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_pack, dim3((W_ELEMS + 255) / 256), dim3(256), 0, c->stream, c->d_params, c->d_sur);
@@ -1072,7 +1087,8 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
     host_tail.residual_tol = s->residual_tol;
     for (int o = 0; o < 16; ++o) {
         host_tail.out_mul[o] = o < NO ? s->out_mul[o] : -1;
-        host_tail.out_log[o] = o < NO ? s->out_log[o] : 0;
+        host_tail.out_kind[o] = o < NO ? s->out_kind[o] : 0;
+        host_tail.out_lin[o] = o < NO ? s->out_lin[o] : 1.0f;
     }
     const size_t off = offsetof(SurDev, in_mean);
     CHEM_TRY(hipMemcpyAsync((char*)c->d_sur + off, (const char*)&host_tail + off, sizeof(SurDev) - off,

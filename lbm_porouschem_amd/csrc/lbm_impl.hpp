@@ -77,11 +77,19 @@ __device__ __forceinline__ unsigned xcd_block() {
     return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
 }
 
+// workgroup size of the bulk kernel (a multiple of 64: each wavefront covers 64 consecutive
+// cells of the fast axis)
+#ifndef LBM_BULK_BLOCK
+#define LBM_BULK_BLOCK 256
+#endif
+constexpr int BULK_BLOCK = LBM_BULK_BLOCK;
+static_assert(BULK_BLOCK % 64 == 0 && BULK_BLOCK <= 1024, "bulk workgroup: whole wavefronts");
+
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
-__global__ __launch_bounds__(BLOCK) LBM_WAVES_PER_EU(M::exact_species ? LBM_BULK_WAVES : 1)
+__global__ __launch_bounds__(BULK_BLOCK) LBM_WAVES_PER_EU(M::exact_species ? LBM_BULK_WAVES : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
-    long long c = (long long)xcd_block() * BLOCK + threadIdx.x;
+    long long c = (long long)xcd_block() * BULK_BLOCK + threadIdx.x;
     if (c >= P.N) return;
     bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }
@@ -995,7 +1003,7 @@ struct ImplT final : Impl {
     }
 
     void launch_bulk(hipStream_t stream, bool odd, int wm) {
-        const dim3 grid(grid_for(N)), block(BLOCK);
+        const dim3 grid((unsigned)((N + BULK_BLOCK - 1) / BULK_BLOCK)), block(BULK_BLOCK);
         if constexpr (D == 3 && NF == 4 && NX == 2) {
             if (coupled) {
                 if (P.coll == COLL_MRT) {

@@ -482,7 +482,8 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
     }
     if (!s->params) return LBM_ERR_INVALID;
     for (int o = 0; o < NO; ++o)
-        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI || (s->out_log[o] != 0 && s->out_log[o] != 1))
+        if (s->out_mul[o] < -1 || s->out_mul[o] >= NI || s->out_kind[o] < 0 || s->out_kind[o] > 3 ||
+            (s->out_kind[o] == 2 && !(s->out_lin[o] > 0.0f)) || (s->out_kind[o] == 3 && o >= NS))
             return LBM_ERR_INVALID;
     memcpy(c->params, s->params, sizeof(c->params));
     c->sur = *s;
@@ -501,10 +502,23 @@ int chem_oracle_surrogate_cell(const lbm_chem* c, const double* in, double dt, i
     }
     mlp_trace t;
     mlp_fwd(c->params, xn, &t);
+    /* decode: output transform by out_kind, per-unit scaling, no gas species below zero */
+    double sym = 0.0;
+    for (int k = 0; k < NS; ++k) sym += in[2 + k] / c->mech.molemass[k];
+    const double rho_g = sym > 0.0 && in[0] > 0.0 ? in[1] / (R_GAS * in[0] * sym) : 0.0;
     for (int o = 0; o < NO; ++o) {
-        out[o] = (double)((float)t.y[o] * s->out_scale[o] + s->out_mean[o]);
-        if (s->out_log[o]) out[o] = -exp(out[o]);
-        if (s->out_mul[o] >= 0) out[o] *= in[s->out_mul[o]];
+        const double z = (double)((float)t.y[o] * s->out_scale[o] + s->out_mean[o]);
+        const double have = o < NS ? rho_g * in[2 + o] : 0.0;
+        double v = z;
+        if (s->out_kind[o] == 1)
+            v = -exp(z);
+        else if (s->out_kind[o] == 2)
+            v = (double)s->out_lin[o] * sinh(z);
+        else if (s->out_kind[o] == 3)
+            v = exp(z) - have;
+        if (s->out_mul[o] >= 0) v *= in[s->out_mul[o]];
+        if (o < NS && v < -have) v = -have;
+        out[o] = v;
     }
     if (!flag && lbm_chem_residual(&c->mech, in, out) > s->residual_tol) flag = 2;
     if (flag && fallback) chem_oracle_cell(&c->mech, in, dt, out);

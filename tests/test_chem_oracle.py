@@ -385,3 +385,48 @@ def test_adam_matches_torch(oracle64):
         Pt.grad = torch.from_numpy(g.copy())
         opt.step()
     np.testing.assert_allclose(P, Pt.detach().numpy(), rtol=1e-5, atol=1e-7)
+
+
+def test_surrogate_output_transforms_invert_training_targets(oracle64):
+    """The decode of lbm_chem_surrogate (out_kind 0/1/2, out_mul) is the inverse of the training
+    targets of chem.surrogate: with the last layer zero the network's output is out_mean, so
+    out_mean = the z targets of one sample must reproduce that sample's integrator outputs."""
+    from lbm_porouschem_amd.chem import surrogate as S
+    rng = np.random.default_rng(11)
+    n = 40
+    X = np.zeros((16, n))
+    X[0] = rng.uniform(900, 1100, n)
+    X[1] = P_ATM * rng.uniform(0.99, 1.01, n)
+    X[2:14] = rng.uniform(0, 0.1, (12, n))
+    X[14] = np.where(np.arange(n) % 4 == 0, 0.0, rng.uniform(0.5, 2.0, n))
+    X[15] = rng.uniform(0, 0.5, n)
+    Y = rng.standard_normal((14, n)) * np.logspace(-12, -4, n)[None, :]   # changes over many decades
+    U = -np.exp(rng.uniform(-30, -5, n))                                   # conversion per unit biomass
+    Y[12] = U * X[14]
+    # species 1 (H) and 4 (OH) as end-of-step partial densities, the rest by the default transforms
+    kinds = list(S.OUT_KIND)
+    kinds[1] = kinds[4] = 3
+    # radicals that reset within the step, to ~1e-9 kg/m^3 (the end-state form is chosen only
+    # for such species: its absolute precision is one FP32 ulp of the partial density held)
+    Y[1] = -0.9 * S.gas_density(X) * X[3] + 1e-9
+    Y[4] = -0.95 * S.gas_density(X) * X[6] + 2e-9
+    Z, kinds, lin = S.training_targets(X, Y, U, out_kind=kinds)
+    assert np.all(lin[:12] > 0) and lin[13] > 0
+    P = np.zeros(chem.NPARAM, dtype=np.float32)   # W3 = 0, b3 = 0: normalised outputs are 0
+    C = chem.Chem(oracle64)
+    for i in range(0, n, 7):
+        sur = {"params": P, "in_mean": np.zeros(16, np.float32), "in_scale": np.ones(16, np.float32),
+               "out_mean": Z[:, i].astype(np.float32), "out_scale": np.ones(14, np.float32),
+               "hull_lo": np.full(16, -1e30, np.float32), "hull_hi": np.full(16, 1e30, np.float32),
+               "residual_tol": 1e300, "out_mul": np.array(S.OUT_MUL, np.int32),
+               "out_kind": np.array(kinds, np.int32), "out_lin": lin}
+        C.set_surrogate(sur)
+        x = np.ascontiguousarray(X[:, i:i + 1].astype(np.float32))
+        y = np.zeros((14, 1), dtype=np.float32)
+        flags = np.zeros(1, dtype=np.int32)
+        C.surrogate(x, y, flags, fallback=0)
+        assert flags[0] == 0
+        ref = Y[:, i].copy()
+        # inputs pass through float32 on the way in, so the per-unit output scales with float32(x)
+        ref[12] = U[i] * float(np.float32(X[14, i]))
+        np.testing.assert_allclose(y[:, 0], ref, rtol=5e-6, atol=0)

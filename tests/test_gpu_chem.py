@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from lbm_porouschem_amd import chem, workloads
+from lbm_porouschem_amd.chem import surrogate as S
 
 pytestmark = pytest.mark.gpu
 
@@ -168,6 +169,39 @@ def test_surrogate_checks_and_fallback(gchem, ochem):
     fb = gf != 0
     p = host(plain).astype(np.float32)
     np.testing.assert_allclose(got[:, fb], p[:, fb], rtol=1e-6, atol=1e-30)
+
+
+def test_surrogate_output_transforms_match_checker(gchem, ochem):
+    """The device decode of the output transforms (asinh-scaled gas and heat outputs, log-space
+    biomass conversion per unit of biomass) against the checker's, cell by cell."""
+    n = 500
+    x = random_states(n, 17)
+    sur = toy_surrogate(x, seed=33)
+    sur["out_kind"] = np.array([2] * 4 + [3] + [2] * 7 + [1, 2], dtype=np.int32)   # OH: end-of-step density
+    sur["out_lin"] = np.array([1e-8] * 12 + [1.0, 1e-3], dtype=np.float32)
+    sur["out_mul"] = np.array([-1] * 12 + [14, -1], dtype=np.int32)
+    sur["out_mean"] = np.array([0.0] * 4 + [-20.0] + [0.0] * 7 + [-14.0, 0.0], dtype=np.float32)
+    sur["out_scale"] = np.ones(14, dtype=np.float32)
+    sur["residual_tol"] = 1e300
+    gchem.set_surrogate(sur)
+    ochem.set_surrogate(sur)
+    xf = x.astype(np.float32)
+    ref = np.zeros((14, n), dtype=np.float32)
+    rflags = np.zeros(n, dtype=np.int32)
+    ochem.surrogate(xf, ref, rflags, dt=1e-5, fallback=0)
+    out = dev(np.zeros((14, n), dtype=np.float32))
+    flags = dev(np.zeros(n, dtype=np.int32))
+    gchem.surrogate(dev(xf), out, flags, dt=1e-5, fallback=0)
+    got, gf = host(out), host(flags)
+    assert (rflags == 0).sum() > 50
+    agree = gf == rflags
+    assert agree.mean() >= 0.98
+    assert np.all(ref[12][xf[14] > 0] < 0) and np.all(ref[12][xf[14] == 0] == 0)
+    # no species below zero: d rho_k >= -rho_k (rho_k from the FP32 inputs the kernels see)
+    rho_k = S.gas_density(xf.astype(np.float64))[None, :] * xf[2:14].astype(np.float64)
+    assert np.all(ref[:12] >= -rho_k * (1 + 1e-6))
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+    assert (np.abs(got - ref) / scale)[:, agree].max() <= 2e-3
 
 
 def coupled_pair(hip, oracle64, precision, shape=(12, 10, 16)):
