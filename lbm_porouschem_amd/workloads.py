@@ -18,6 +18,13 @@ from .flag import BC, BC_FLOW, FORCE_TERM, SOURCE_TERM
 R_GAS = 8.314441
 
 
+def extent(n, dx):
+    """Physical length that the solver turns back into n cells: the constructor takes
+    int(X / dx) (LBM3D/_core.py), and n * dx / dx can land just below n in floating point
+    (375 * 1e-4 / 1e-4 = 374.99999999999994), so the length is padded by one part in 1e9."""
+    return n * dx * (1.0 + 1e-9)
+
+
 def gray_packing(shape, r_lo, r_hi, coverage, seed, solid_value):
     """Overlapping random disks (2D, shape[2] == 1) or spheres placed until the
     covered fraction reaches `coverage`; covered cells get `solid_value`.
@@ -51,7 +58,7 @@ def plain2d(lib=None, nx=256, ny=128, seed=2024, precision="fp64"):
     outflow, walls T = 0."""
     from .solver import LBM2DSolver
     dx = dt = 1e-4
-    lb = LBM2DSolver(nx * dx, ny * dx, 1, dx=dx, dt=dt, name="plain2d", isThermal=True, isChemical=False,
+    lb = LBM2DSolver(extent(nx, dx), extent(ny, dx), 1, dx=dx, dt=dt, name="plain2d", isThermal=True, isChemical=False,
                      isPoro=True, isRadiation=False, precision=precision, lib=lib)
     lb.source_term_model = SOURCE_TERM.MICRO
     lb.force_term_model = FORCE_TERM.GUO
@@ -115,7 +122,7 @@ def coupled3d(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", col
     inlet T = 1 (800 K)."""
     from .solver import LBM3DSolver
     dx = dt = 1e-4
-    lb = LBM3DSolver(nx * dx, ny * dx, nz * dx, dx=dx, dt=dt, name="coupled3d", isThermal=True, isChemical=True,
+    lb = LBM3DSolver(extent(nx, dx), extent(ny, dx), extent(nz, dx), dx=dx, dt=dt, name="coupled3d", isThermal=True, isChemical=True,
                      isPoro=True, isRadiation=False, precision=precision, collision=collision, lib=lib)
     # int(X/dx) may truncate a float product; pin the requested shape
     assert (lb.nx, lb.ny, lb.nz) == (nx, ny, nz), (lb.nx, lb.ny, lb.nz)
@@ -172,7 +179,7 @@ def coupled2d(lib=None, nx=256, ny=256, seed=2025, precision="fp32", solid_value
     biomass/char, gray disks (radius 4-8 lu), three competing pyrolysis reactions, heated inlet."""
     from .solver import LBM2DSolver
     dx = dt = 1e-4
-    lb = LBM2DSolver(nx * dx, ny * dx, 1, dx=dx, dt=dt, name="coupled2d", isThermal=True, isChemical=True,
+    lb = LBM2DSolver(extent(nx, dx), extent(ny, dx), 1, dx=dx, dt=dt, name="coupled2d", isThermal=True, isChemical=True,
                      isPoro=True, isRadiation=False, precision=precision, lib=lib)
     assert (lb.nx, lb.ny) == (nx, ny), (lb.nx, lb.ny)
     lb.source_term_model = SOURCE_TERM.MICRO
@@ -236,7 +243,7 @@ def pyrolysis12(lib=None, nx=256, ny=256, nz=256, seed=2025, precision="fp32", c
     from . import chem as chemmod
     from .solver import LBM3DSolver
     dx, dt = 1e-4, 1e-5
-    lb = LBM3DSolver(nx * dx, ny * dx, nz * dx, dx=dx, dt=dt, name="pyrolysis12", isThermal=True, isChemical=True,
+    lb = LBM3DSolver(extent(nx, dx), extent(ny, dx), extent(nz, dx), dx=dx, dt=dt, name="pyrolysis12", isThermal=True, isChemical=True,
                      isPoro=True, isRadiation=False, precision=precision, collision=collision, lib=lib)
     assert (lb.nx, lb.ny, lb.nz) == (nx, ny, nz), (lb.nx, lb.ny, lb.nz)
     lb.source_term_model = SOURCE_TERM.MICRO
