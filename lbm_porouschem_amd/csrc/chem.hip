@@ -421,7 +421,7 @@ struct SrcNorm {
 // attached lattice handle: cell fields in, lattice-unit sources out
 template <class Real> struct SrcFields {
     lbmimpl::ChemFields<Real> f;
-    const double* molemass;  // device copy of the mechanism's molar masses
+    const double* inv_molemass;  // device copy of 1 / the mechanism's molar masses
     __device__ bool load(long long i, double* x) const {
         const long long N = f.N;
         const Real sol = f.solid[i];
@@ -442,7 +442,7 @@ template <class Real> struct SrcFields {
         const long long N = f.N;
         double sym = 0.0;
 #pragma unroll
-        for (int q = 0; q < NS; ++q) sym += x[2 + q] / molemass[q];
+        for (int q = 0; q < NS; ++q) sym += x[2 + q] * inv_molemass[q];
         const double rho_g = sym > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
         double tot = 0.0, dsum = 0.0;
 #pragma unroll
@@ -450,9 +450,10 @@ template <class Real> struct SrcFields {
             tot += rho_g * x[2 + q] + y[q];
             dsum += y[q];
         }
+        const double inv_tot = tot > 0.0 ? 1.0 / tot : 0.0;
 #pragma unroll
         for (int q = 0; q < NS; ++q)
-            f.xsrc[(long long)q * N + i] = (Real)(tot > 0.0 ? (rho_g * x[2 + q] + y[q]) / tot - x[2 + q] : 0.0);
+            f.xsrc[(long long)q * N + i] = (Real)(tot > 0.0 ? (rho_g * x[2 + q] + y[q]) * inv_tot - x[2 + q] : 0.0);
         f.xsrc[12 * N + i] = (Real)(y[12] / f.rho_ref);
         f.xsrc[13 * N + i] = (Real)((-y[12] - dsum) / f.rho_ref);
         const bool sol = f.solid[i] > (Real)0;
@@ -557,6 +558,11 @@ struct SurDev {
     int out_mul[16];
     int out_kind[16];
     float out_lin[16];
+    // reciprocal tables of the mechanism for the per-cell checks (multiplications, no FP64 division)
+    double inv_M[NS];          // 1 / M_k
+    double eM[4][NS];          // atoms of element e per molecule of k / M_k
+    double bio_e[4];           // atoms of element e per biomass unit / M_biomass
+    double inv_MC;             // 1 / M_char
 };
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -611,20 +617,53 @@ __device__ __forceinline__ bf16x8 frag8(const unsigned short* wl, int base, int 
 
 // Physical outputs of one cell from the network's normalised outputs yn (lbm_surrogate: output
 // transform by out_kind, per-unit scaling by out_mul, no gas species below zero).
-__device__ void decode(const SurDev& S, const lbm_chem_mech& m, const double* x, const float* yn, double* y) {
-    double sym = 0.0;
-    for (int k = 0; k < NS; ++k) sym += x[2 + k] / m.molemass[k];
-    const double rho_g = sym > 0.0 && x[0] > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
+// The transforms run in FP32 (the network's output is FP32); the species floor in FP64.
+__device__ void decode(const SurDev& S, const double* x, const float* yn, double rho_g, double* y) {
 #pragma unroll
     for (int o = 0; o < NO; ++o) {
-        const double z = (double)(yn[o] * S.out_scale[o] + S.out_mean[o]);
+        const float z = yn[o] * S.out_scale[o] + S.out_mean[o];
         const double have = o < NS ? rho_g * x[2 + o] : 0.0;
         const int kd = S.out_kind[o];
-        double v = kd == 1 ? -exp(z) : (kd == 2 ? (double)S.out_lin[o] * sinh(z) : (kd == 3 ? exp(z) - have : z));
+        double v = kd == 1 ? -(double)expf(z)
+                           : (kd == 2 ? (double)(S.out_lin[o] * sinhf(z)) : (kd == 3 ? (double)expf(z) - have : (double)z));
         if (S.out_mul[o] >= 0) v *= x[S.out_mul[o]];
         if (o < NS && v < -have) v = -have;
         y[o] = v;
     }
+}
+
+// gas density of a cell's inputs, rho_g = p / (R T sum_k Y_k / M_k)
+__device__ double gas_density(const SurDev& S, const double* x) {
+    double sym = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) sym += x[2 + k] * S.inv_M[k];
+    return sym > 0.0 && x[0] > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
+}
+
+// element-balance residual of lbm_chem_residual with the reciprocal tables
+__device__ double residual_fast(const SurDev& S, const double* in, const double* out, double rho_g) {
+    const double B = in[NS + 2];
+    double dsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) dsum += out[k];
+    const double dB = out[NS], dC = -dB - dsum;
+    double inv = 0.0, worst = 0.0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        double gas = 0.0, have = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            gas += S.eM[e][k] * out[k];
+            have += S.eM[e][k] * rho_g * in[2 + k];
+        }
+        have += B * S.bio_e[e];
+        double rel = -dB * S.bio_e[e];
+        if (e == 0) rel -= dC * S.inv_MC;
+        inv += have;
+        const double d = fabs(gas - rel);
+        if (d > worst) worst = d;
+    }
+    return inv > 0.0 ? worst / inv : worst;
 }
 
 // Row stride (floats) of the per-wavefront cell rows in LDS: odd, so that lane l writing
@@ -672,8 +711,10 @@ __device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __r
 // Surrogate over n cells, SUR_BLOCK cells per block iteration (64 per wavefront: every lane
 // gathers and checks one cell; the wavefront's MLP runs as four 16-cell tiles).
 // CHECK = 0: normalised in/out (lbm_mlp_forward).
+// two blocks of 8 wavefronts per CU fit in LDS, i.e. 4 wavefronts per SIMD: ask the register
+// allocator for that occupancy (at most 128 VGPRs)
 template <class Src, int CHECK>
-__global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restrict__ S,
+__global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_surrogate(const SurDev* __restrict__ S,
                                                          const lbm_chem_mech* __restrict__ mech, Src src, long long n,
                                                          unsigned* list, unsigned* count,
                                                          unsigned long long* stats) {
@@ -720,11 +761,12 @@ __global__ __launch_bounds__(SUR_BLOCK) void k_surrogate(const SurDev* __restric
         if (act) {
             if constexpr (CHECK) {
                 double y[NO];
-                decode(*S, *mech, x, buf + l * XS, y);
+                const double rho_g = gas_density(*S, x);
+                decode(*S, x, buf + l * XS, rho_g, y);
 #pragma unroll
                 for (int v = 0; v < NI; ++v)
                     if (x[v] < (double)S->lo[v] || x[v] > (double)S->hi[v]) hull = true;
-                if (!hull && residual(*mech, x, y) > S->residual_tol) res = true;
+                if (!hull && residual_fast(*S, x, y, rho_g) > S->residual_tol) res = true;
                 src.predicted(cell, x, y, hull ? 1 : (res ? 2 : 0));
             } else {
 #pragma unroll
@@ -881,7 +923,7 @@ struct lbm_chem {
     hipStream_t stream = nullptr;
     lbm_chem_mech mech{};
     lbm_chem_mech* d_mech = nullptr;
-    double* d_molemass = nullptr;
+    double* d_molemass = nullptr;  // device copy of 1 / molemass (the per-cell passes multiply)
     SurDev* d_sur = nullptr;    // loaded surrogate
     SurDev* d_tmp = nullptr;    // scratch for lbm_mlp_forward
     float* d_params = nullptr;  // staging of host parameters
@@ -1016,7 +1058,9 @@ int lbm_chem_create(const lbm_chem_mech* mech, int device, lbm_chem** out) {
         if (hipMalloc(&c->d_params, NPARAM * sizeof(float)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMalloc(&c->d_count, 2 * sizeof(unsigned)) != hipSuccess) { rc = LBM_ERR_OOM; break; }
         if (hipMemcpy(c->d_mech, mech, sizeof(lbm_chem_mech), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
-        if (hipMemcpy(c->d_molemass, mech->molemass, NS * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
+        double inv_m[NS];
+        for (int k = 0; k < NS; ++k) inv_m[k] = 1.0 / mech->molemass[k];
+        if (hipMemcpy(c->d_molemass, inv_m, NS * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { rc = LBM_ERR_DEVICE; break; }
     } while (0);
     if (rc != LBM_OK) {
         lbm_chem_destroy(c);
@@ -1085,6 +1129,12 @@ int lbm_chem_set_surrogate(lbm_chem* c, const lbm_surrogate* s) {
         host_tail.out_scale[i] = i < NO ? s->out_scale[i] : 0.0f;
     }
     host_tail.residual_tol = s->residual_tol;
+    for (int k = 0; k < NS; ++k) {
+        host_tail.inv_M[k] = 1.0 / c->mech.molemass[k];
+        for (int e = 0; e < 4; ++e) host_tail.eM[e][k] = c->mech.elements[k][e] / c->mech.molemass[k];
+    }
+    for (int e = 0; e < 4; ++e) host_tail.bio_e[e] = c->mech.biomass_elements[e] / c->mech.biomass_molemass;
+    host_tail.inv_MC = 1.0 / c->mech.char_molemass;
     for (int o = 0; o < 16; ++o) {
         host_tail.out_mul[o] = o < NO ? s->out_mul[o] : -1;
         host_tail.out_kind[o] = o < NO ? s->out_kind[o] : 0;

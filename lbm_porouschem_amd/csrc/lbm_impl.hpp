@@ -101,15 +101,20 @@ void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int wri
 // fast axis fastest (coalesced side-buffer rows).
 // The fast-axis faces (z in 3D, y in 2D) run over `rows`, the list of rows whose
 // face cells are not handled inside the bulk kernel.
+// work items of one NEE family
+template <int D>
+__host__ __device__ inline long long nee_items(int family, int nx, int ny, int nz, int nparts, long long nrows) {
+    const bool fastfam = family == D - 1;
+    const long long ncell = fastfam ? nrows : (family == 0 ? (long long)ny * nz : (long long)nx * nz);
+    return ncell * 2 * nparts;
+}
+
 template <class Real, class Store, int D, int NF, int NX>
-__global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int family,
-                                               int nparts, const uint32_t* __restrict__ rows, long long nrows) {
-    const DevParams<Real>& P = *Pp;
-    const long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+__device__ void nee_item(const DevParams<Real>& P, const Bufs<Real, Store>& B, int family, int nparts,
+                         const uint32_t* __restrict__ rows, long long nrows, long long t) {
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const bool fastfam = family == (D == 3 ? 2 : 1);
     const long long ncell = fastfam ? nrows : (family == 0 ? (long long)ny * nz : (long long)nx * nz);
-    if (t >= ncell * 2 * nparts) return;
     const long long cell = t % ncell;
     const int side = (int)((t / ncell) % 2);
     const int part = (int)(t / (ncell * 2));
@@ -157,6 +162,15 @@ __global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict
     nee_part<Real, Store, D, NF, NX>(P, B, kind, face, bx, by, bz, ix, iy, iz, fc);
 }
 
+template <class Real, class Store, int D, int NF, int NX>
+__global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int family,
+                                               int nparts, const uint32_t* __restrict__ rows, long long nrows) {
+    const DevParams<Real>& P = *Pp;
+    const long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= nee_items<D>(family, P.nx, P.ny, P.nz, nparts, nrows)) return;
+    nee_item<Real, Store, D, NF, NX>(P, B, family, nparts, rows, nrows, t);
+}
+
 template <class Real, class Store, int D, int NF, int NX, int PAR>
 __global__ __launch_bounds__(BLOCK) void k_face(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B) {
     const DevParams<Real>& P = *Pp;
@@ -164,6 +178,7 @@ __global__ __launch_bounds__(BLOCK) void k_face(const DevParams<Real>* __restric
     if (t >= B.nflist) return;
     face_collide_cell<Real, Store, D, NF, NX, PAR>(P, B, t);
 }
+
 
 // 2D inflow-rate controller: one block sums rho*u_x over the inlet column in a
 // fixed order, then rescales the profile from column x = 1.

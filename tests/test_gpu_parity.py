@@ -173,3 +173,4 @@ def test_plain2d_full_run_matches_checker(hip, oracle64):
     # the run developed a flow through the packing and heated it
     assert B.get_max_v() > 0.01
     assert np.abs(fb["TS"]).max() > 0.0
+
