@@ -1,0 +1,455 @@
+"""Pins the CPU restatement's property models (oracle/lbm_oracle.c) against closed forms.
+
+Each model is restated here from the reference's source, not from the oracle, and checked
+through an observable whose closed form fixes the coefficient the model feeds:
+
+  Sutherland viscosity        shear-wave decay exp(-nu k^2 t), nu = A T^1.5/(T + T_s) dt/dx^2 / rho
+                              (viscosity, LBM3D/_evolution.py:227-241)
+  MIXTURE viscosity           nu = 0.1 + sum_j Y_j nu_j (the same lines; Specie.viscosity,
+                              LBM3D/_chemical.py:70-77)
+  Schmidt diffusivity         Gaussian variance growth 2 D per step, D = Sc nu_j dt/dx^2 with
+                              nu_j already in lattice units (Specie.coefDiff, _chemical.py:62-68)
+  DERIVED thermal diffusivity D = kappa / rho / c_m dt/dx^2 with POLYNOMIAL, NASA7 and MIXTURE
+                              kappa and c_m (TemperatureFluid, LBM3D/_thermal.py:62-106;
+                              TemperatureSolid, :169-211)
+  Darcy-Forchheimer drag      uniform-flow momentum recursion with the DF force and velocity
+                              correction (LBM3D/_evolution.py:126-150, 266-277)
+  INCOMPRESSIBLE equilibrium  zeroth to second moments of f_eq (LBM3D/_evolution.py:213-225)
+  2D inflow-rate controller   inlet profile after one step (LBM2D/_boundary.py:324-333)
+
+Lattice units differ from physical ones here (dx = 1e-3, dt = 1e-4, dt/dx^2 = 100), so a
+dropped or doubled dt/dx^2 factor fails; each test's numbers put the result far from what a
+plausible slip (a wrong exponent, a missing density, a swapped Schmidt factor) would give.
+"""
+import numpy as np
+import pytest
+
+from lbm_porouschem_amd.flag import BC, BC_FLOW, BC_MODEL, FLUID_STATE_EQUATION, FORCE_TERM
+from lbm_porouschem_amd.solver import LBM2DSolver, LBM3DSolver
+
+DX, DT = 1e-3, 1e-4
+DTDX2 = DT / DX ** 2     # 100
+R_REF = 8.314441          # util/constant.py:17
+
+
+def _solver(oracle, dim, n, dx=DX, dt=DT, **kw):
+    if dim == 2:
+        return LBM2DSolver(n[0] * dx * (1 + 1e-9), n[1] * dx * (1 + 1e-9), 1, dx=dx, dt=dt, precision="fp64",
+                           lib=oracle, **kw)
+    return LBM3DSolver(n[0] * dx * (1 + 1e-9), n[1] * dx * (1 + 1e-9), n[2] * dx * (1 + 1e-9), dx=dx, dt=dt,
+                       precision="fp64", lib=oracle, **kw)
+
+
+# ---------------------------------------------------------------------------------------------
+# viscosity models: shear-wave decay
+# ---------------------------------------------------------------------------------------------
+def _shear_nu(lb, ny, t1=40, t2=240, U=1e-4):
+    """Measured kinematic viscosity of u_x = U sin(2 pi y / ny) between steps t1 and t2."""
+    shape = (lb.nx, lb.ny, lb.nz)
+    y = np.arange(ny)
+    prof = np.sin(2 * np.pi * y / ny)
+    lb.init_field(lb.rho, np.ones(shape))
+    lb.init_field3(lb.v, np.broadcast_to(U * prof[None, :, None], shape).copy(), np.zeros(shape), np.zeros(shape))
+    lb.init_simulation()
+    amps = []
+    for t in (t1, t2):
+        lb.run(t - lb.tLattice)
+        u = lb.v.to_numpy()[..., 0].mean(axis=(0, 2))
+        amps.append(2 / ny * (u * prof).sum())
+    k = 2 * np.pi / ny
+    return -np.log(amps[1] / amps[0]) / (k * k * (t2 - t1))
+
+
+def _thermal_uniform(lb, T_phys, Trange=(300.0, 900.0)):
+    lb.set_fluid_Trange(list(Trange))
+    lb.set_fluid_thermal_diff(1e-4)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    shape = (lb.nx, lb.ny, lb.nz)
+    lb.init_field(lb.TF.S, np.full(shape, (T_phys - Trange[0]) / (Trange[1] - Trange[0])))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sutherland_viscosity_sets_shear_decay(oracle64, dim):
+    ny = 32
+    n = (4, ny) if dim == 2 else (4, ny, 4)
+    T, Ts = 600.0, 110.4
+    nu_target = 0.05
+    A = nu_target / (T ** 1.5 / (T + Ts) * DTDX2)
+    lb = _solver(oracle64, dim, n, isThermal=True)
+    lb.set_viscosity_sutherland(A, Ts)
+    _thermal_uniform(lb, T)
+    nu = _shear_nu(lb, ny)
+    # rho = 1 (IDEAL_GAS: kinematic = dynamic / rho); T^0.5 instead of T^1.5 or a dropped dt/dx^2
+    # would be off by orders of magnitude, a T + Ts -> T slip by 18 %
+    assert nu == pytest.approx(nu_target, rel=0.01)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mixture_viscosity_sets_shear_decay(oracle64, dim):
+    ny = 32
+    n = (4, ny) if dim == 2 else (4, ny, 4)
+    lb = _solver(oracle64, dim, n, isThermal=True, isChemical=True)
+    lb.set_viscosity_mixture()
+    lb.set_specie("A", False, 28.0)
+    lb.set_specie("B", False, 44.0)
+    nuA, nuB = 2e-4, 1e-3              # physical; lattice units x dt/dx^2 = 0.02, 0.1
+    lb.set_specie_viscosity("A", nuA)
+    lb.set_specie_viscosity("B", nuB)
+    for s in ("A", "B"):
+        lb.set_specie_diff(s, 1e-4)
+    YA, YB = 0.3, 0.7
+    lb.init_specie("A", YA)
+    lb.init_specie("B", YB)
+    _thermal_uniform(lb, 600.0)
+    nu = _shear_nu(lb, ny)
+    # viscosity starts at 0.1 (lattice units) and adds Y_j nu_j (Specie.viscosity in lattice units)
+    assert nu == pytest.approx(0.1 + YA * nuA * DTDX2 + YB * nuB * DTDX2, rel=0.01)
+
+
+# ---------------------------------------------------------------------------------------------
+# diffusivity models: Gaussian pulse variance
+# ---------------------------------------------------------------------------------------------
+def _pulse(nx, base, amp, sig0=5.0):
+    x = np.arange(nx)
+    return x, base + amp * np.exp(-(x - nx / 2) ** 2 / (2 * sig0 ** 2)), sig0
+
+
+def _expected_variance(sig0, D, n, dim):
+    """Exact second-moment recursion of the scalar BGK lattice from an equilibrium start (no
+    flow, periodic, pulse away from the edges): with A_t = sum_x x j_t and c2 = c_s^2
+    (1/3 for D2Q5, 1/4 for D3Q7), tau = D / c2 + 1/2 and w = 1 - 1/tau,
+      A_{t+1} = w A_t + c2 M,   V_{t+1} = V_t + 2 w A_t + c2 M,   A_0 = 0,
+    so  var_n - var_0 = 2 D n - 2 w tau^2 c2 (1 - w^n)."""
+    c2 = 1 / 3 if dim == 2 else 1 / 4
+    tau = D / c2 + 0.5
+    w = 1 - 1 / tau
+    return sig0 ** 2 + 2 * D * n - 2 * w * tau ** 2 * c2 * (1 - w ** n)
+
+
+def _variance(field1d, base, nx):
+    x = np.arange(nx)
+    T = field1d - base
+    return ((x - nx / 2) ** 2 * T).sum() / T.sum()
+
+
+def test_schmidt_species_diffusivity_scaling(oracle64):
+    """D = Sc * nu_j * dt/dx^2 with nu_j = visco * dt/dx^2 (both factors, as the reference
+    writes them); dx = dt = 0.1 gives dt/dx^2 = 10, so a single factor would give D / 10."""
+    nx, steps = 160, 200
+    dx = dt = 0.1
+    lb = _solver(oracle64, 2, (nx, 4), dx=dx, dt=dt, isThermal=False, isChemical=True)
+    nuA, Sc = 1e-3, 1.5            # nu_j = 1e-3 * 10 = 0.01 (lattice); D = 1.5 * 0.01 * 10 = 0.15
+    # the macro step divides the species by their sum (LBM3D/_evolution.py:162-172): a second
+    # species B = 1 - A with the same diffusivity keeps that sum at exactly 1
+    for name in ("A", "B"):
+        lb.set_specie(name, False, 28.0)
+        lb.set_specie_viscosity(name, nuA)
+        lb.set_specie_diff_Schmidt(name, Sc)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    x, prof, sig0 = _pulse(nx, 0.1, 0.5)
+    A = np.broadcast_to(prof[:, None, None], (lb.nx, lb.ny, lb.nz)).copy()
+    lb.init_specie("A", A)
+    lb.init_specie("B", 1.0 - A)
+    lb.init_simulation()
+    lb.run(steps)
+    var = _variance(lb.species[0].S.to_numpy()[:, 0, 0], 0.1, nx)
+    D = Sc * (nuA * dt / dx ** 2) * dt / dx ** 2
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+def _poly5_expanded(c, T):
+    """The reference's c0+T*(c1+T*(c2*T+T*(c3+T*c4))) (LBM3D/_thermal.py:78, 92), expanded:
+    c0 + c1 T + (c2 + c3) T^3 + c4 T^4."""
+    return c[0] + c[1] * T + (c[2] + c[3]) * T ** 3 + c[4] * T ** 4
+
+
+RHO = 1.3   # fluid density of the DERIVED cases: D = kappa / rho / c_m (rho = 1 would hide it)
+
+
+def _tf_pulse_variance(lb, T0, Trange, steps=200, nx=160, amp_K=0.01):
+    vs = Trange[1] - Trange[0]
+    base = (T0 - Trange[0]) / vs
+    x, prof, sig0 = _pulse(nx, base, amp_K / vs)
+    lb.set_fluid_Trange(list(Trange))
+    lb.boundary_condition_model = BC_MODEL.NONE
+    lb.init_field(lb.rho, RHO)
+    lb.init_field(lb.TF.S, np.broadcast_to(prof[:, None, None], (lb.nx, lb.ny, lb.nz)).copy())
+    lb.init_simulation()
+    lb.run(steps)
+    return _variance(lb.TF.S.to_numpy()[:, 0, 0], base, nx), sig0
+
+
+def test_derived_diffusivity_polynomial_conductivity_and_capacity(oracle64):
+    nx, steps, T0 = 160, 200, 700.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True)
+    kc = [0.28, 1.4e-4, 2.8e-10, 1.4e-10, 1.4e-13]  # W/(m K)
+    cc = [900.0, 0.2, 1e-7, 1e-7, 0.0]          # J/(kg K)
+    lb.set_fluid_thermal_diff_derived()
+    lb.set_fluid_conductivity_poly(kc)
+    lb.set_fluid_capacity_poly(cc)
+    var, sig0 = _tf_pulse_variance(lb, T0, (300.0, 1100.0), steps, nx)
+    D = _poly5_expanded(kc, T0) / RHO / _poly5_expanded(cc, T0) * DTDX2
+    assert 0.005 < D < 0.5
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+def test_derived_diffusivity_nasa7_capacity(oracle64):
+    """Fluid NASA7 capacity: the low-range polynomial below Trange[1], times R."""
+    nx, steps, T0 = 160, 200, 700.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True)
+    low = [3.5, 1e-4, 2e-8, 0.0, 0.0, 0.0, 0.0]
+    high = [2.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    lb.set_fluid_thermal_diff_derived()
+    lb.set_fluid_conductivity(0.05)
+    lb.set_fluid_capacity_NASA7([300.0, 1000.0, 3000.0], [low, high])
+    var, sig0 = _tf_pulse_variance(lb, T0, (300.0, 1100.0), steps, nx)
+    cm = _poly5_expanded(low, T0) * R_REF
+    D = 0.05 / RHO / cm * DTDX2
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+def test_derived_diffusivity_mixture_conductivity_and_capacity(oracle64):
+    """MIXTURE: kappa = sum_j kappa_j Y_j, c_m = sum_j (capa_j / M_j) Y_j over fluid species,
+    with M_j in kg/mol (the molar mass is given in g/mol)."""
+    nx, steps, T0 = 160, 200, 700.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True, isChemical=True)
+    lb.set_specie("A", False, 28.0)
+    lb.set_specie("B", False, 44.0)
+    for s, kap, cap in (("A", 0.04, 29.0), ("B", 0.06, 37.0)):
+        lb.set_specie_conductivity(s, kap)
+        lb.set_specie_capacity(s, cap)
+        lb.set_specie_diff(s, 1e-4)
+    YA, YB = 0.25, 0.75
+    lb.init_specie("A", YA)
+    lb.init_specie("B", YB)
+    lb.set_fluid_thermal_diff_derived()
+    lb.set_fluid_conductivity_mixture()
+    lb.set_fluid_capacity_mixture()
+    var, sig0 = _tf_pulse_variance(lb, T0, (300.0, 1100.0), steps, nx)
+    kappa = 0.04 * YA + 0.06 * YB
+    cm = 29.0 / 0.028 * YA + 37.0 / 0.044 * YB
+    D = kappa / RHO / cm * DTDX2
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+def test_derived_solid_diffusivity_polynomials(oracle64):
+    """Solid DERIVED diffusivity: kappa_s(T) = c0+T*(c1+T*(c2*T+T*c3)) (4 terms,
+    LBM3D/_thermal.py:177-189) = c0 + c1 T + (c2 + c3) T^3; divided by rho_s and the 5-term
+    capacity polynomial."""
+    nx, steps, T0 = 160, 200, 650.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True, isPoro=True)
+    shape = (lb.nx, lb.ny, lb.nz)
+    lb.add_solid(np.full(shape, 0.5))
+    rhos = 3.0
+    lb.add_rho_solid(np.full(shape, rhos))
+    lb.set_viscosity(1e-4)
+    lb.set_fluid_thermal_diff(1e-4)
+    lb.set_fluid_Trange([300.0, 1100.0])
+    lb.set_fluid_capacity(1000.0)
+    lb.set_solid_thermal_diff_derived()
+    ks = [0.5, 1e-4, 3e-10, 1e-10]
+    cs = [700.0, 0.5, 0.0, 0.0, 0.0]
+    lb.set_solid_conductivity_poly(ks)
+    lb.set_solid_capacity_poly(cs)
+    Trange = (300.0, 1100.0)
+    vs = Trange[1] - Trange[0]
+    base = (T0 - Trange[0]) / vs
+    x, prof, sig0 = _pulse(nx, base, 0.01 / vs)
+    lb.set_solid_Trange(list(Trange))
+    lb.boundary_condition_model = BC_MODEL.NONE
+    lb.init_field(lb.TF.S, np.full(shape, base))
+    lb.init_field(lb.TS.S, np.broadcast_to(prof[:, None, None], shape).copy())
+    lb.init_simulation()
+    lb.run(steps)
+    var = _variance(lb.TS.S.to_numpy()[:, 0, 0], base, nx)
+    kappa = ks[0] + ks[1] * T0 + (ks[2] + ks[3]) * T0 ** 3
+    D = kappa / rhos / _poly5_expanded(cs, T0) * DTDX2
+    assert 0.005 < D < 0.5
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+# ---------------------------------------------------------------------------------------------
+# Darcy-Forchheimer drag: uniform-flow recursion
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_darcy_forchheimer_uniform_recursion(oracle64, dim):
+    """Uniform fields (streaming is the identity).  Raw momentum m, stored velocity u:
+      collision  m' = m - (m - rho u)/tau + (1 - 1/(2 tau)) rho F(u),
+      F(u) = (-eps nu_F C_D - eps C_F |u|) u  (nu_F dynamic in 3D, kinematic in 2D, D14),
+      macro      u' = (m'/rho) / (c0 + sqrt(c0^2 + c1 |m'/rho|)),
+                 c0 = (1 + eps nu_k C_D / 2) / 2,  c1 = eps C_F / 2,
+    with C_D = coefDarcy dx^2 and C_F = coefForchheimer dx (LBM3D/_io.py:152-171)."""
+    n = (6, 5) if dim == 2 else (5, 4, 4)
+    dx = dt = 1.0
+    lb = _solver(oracle64, dim, n, dx=dx, dt=dt, isPoro=True)
+    shape = (lb.nx, lb.ny, lb.nz)
+    sol, cd, cf, nu, rho = 0.35, 0.2, 3.0, 0.1, 1.3
+    lb.add_solid(np.full(shape, sol))
+    lb.force_term_model = FORCE_TERM.GUO
+    lb.set_viscosity(nu)
+    lb.set_poro_Darcy_Forchheimer(np.ones(shape), cd, cf)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    u0 = np.array([0.03, -0.02, 0.01 if dim == 3 else 0.0])
+    lb.init_field(lb.rho, np.full(shape, rho))
+    lb.v.from_numpy(np.broadcast_to(u0, shape + (3,)).copy())
+    lb.init_simulation()
+    steps = 25
+    lb.run(steps)
+    eps = 1 - sol
+    nu_k = nu / rho
+    tau = 3 * nu_k + 0.5
+    nu_F = nu if dim == 3 else nu_k
+    m = rho * u0.copy()
+    u = u0.copy()
+    for _ in range(steps):
+        F = (-eps * nu_F * cd - eps * cf * np.linalg.norm(u)) * u
+        m = m - (m - rho * u) / tau + (1 - 1 / (2 * tau)) * rho * F
+        raw = m / rho
+        c0 = 0.5 * (1 + eps * nu_k / 2 * cd)
+        c1 = 0.5 * eps * cf
+        u = raw / (c0 + np.sqrt(c0 ** 2 + c1 * np.linalg.norm(raw)))
+    assert np.allclose(lb.v.to_numpy(), u, rtol=1e-11, atol=1e-16)
+    # the quadratic (Forchheimer) part matters: without it the velocity would differ
+    assert np.linalg.norm(u) < 0.5 * np.linalg.norm(u0)
+
+
+# ---------------------------------------------------------------------------------------------
+# INCOMPRESSIBLE equilibrium moments
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_incompressible_equilibrium_moments(oracle64, dim):
+    """f_eq = w (rho + 3 e.u + 4.5 (e.u)^2/eps - 1.5 u^2/eps) (eps + 1e-12 in the denominators):
+    sum f = rho, sum e f = u (not rho u), sum e_a e_b f = rho/3 delta_ab + u_a u_b / eps."""
+    from test_oracle_pinning import _lattice, _pops
+    n = (5, 4) if dim == 2 else (4, 4, 4)
+    lb = _solver(oracle64, dim, n, dx=1.0, dt=1.0, isPoro=True)
+    lb.EOS = FLUID_STATE_EQUATION.INCOMPRESSIBLE
+    shape = (lb.nx, lb.ny, lb.nz)
+    sol = 0.3
+    lb.add_solid(np.full(shape, sol))
+    lb.set_viscosity(0.1)
+    lb.boundary_condition_model = BC_MODEL.NONE
+    rho, u = 1.2, np.array([0.04, -0.03, 0.02 if dim == 3 else 0.0])
+    lb.init_field(lb.rho, np.full(shape, rho))
+    lb.v.from_numpy(np.broadcast_to(u, shape + (3,)).copy())
+    lb.init_simulation()
+    e, w, lr = _lattice(lb, 0)
+    f = np.array([_pops(lb, 0, k)[0, 0, 0] for k in range(len(w))])
+    eps = 1 - sol + 1e-12
+    assert f.sum() == pytest.approx(rho, rel=1e-13)
+    np.testing.assert_allclose(e.T @ f, u, rtol=1e-12, atol=1e-15)
+    second = np.einsum("ka,kb,k->ab", e, e, f)
+    expect = rho / 3 * np.eye(3) + np.outer(u, u) / eps
+    if dim == 2:
+        expect[2, 2] = 0.0
+    np.testing.assert_allclose(second, expect, rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------------------------
+# 2D inflow-rate controller
+# ---------------------------------------------------------------------------------------------
+def test_inflow_rate_controller_profile(oracle64):
+    """Before each step the inlet profile becomes (0.01 coef + 0.99) u(x = 1, y) with
+    coef = flow_BC / sum_y rho(0, y) u_x(0, y); the NEE then sets the inlet velocity to it."""
+    nx, ny = 24, 16
+    lb = _solver(oracle64, 2, (nx, ny), dx=1.0, dt=1.0)
+    lb.set_viscosity(0.1)
+    lb.set_BC(0, BC_FLOW.inlet_flow)
+    lb.set_flow_BC_value(0, 0.5)
+    lb.set_v_BC_value(0, [0.02, 0.0, 0.0])
+    lb.set_BC(1, BC_FLOW.outlet)
+    lb.set_rho_BC_value(1, 1.0)
+    lb.set_BC(2, BC_FLOW.wall)
+    lb.set_BC(3, BC_FLOW.wall)
+    lb.init_field(lb.rho, 1.0)
+    lb.init_simulation()
+    for _ in range(5):
+        lb.run(1)
+        v = lb.v.to_numpy()[:, :, 0]
+        rho = lb.rho.to_numpy()[:, :, 0]
+        coef = 0.5 / (rho[0, :] * v[0, :, 0]).sum()
+        profile = (0.01 * coef + 0.99) * v[1, :, :]
+        lb.run(1)
+        v2 = lb.v.to_numpy()[:, :, 0]
+        # inlet cells away from the corners (the wall faces come after the x faces, D9)
+        np.testing.assert_allclose(v2[0, 1:-1, :], profile[1:-1, :], rtol=1e-12, atol=1e-15)
+
+
+def test_derived_solid_diffusivity_nasa7_capacity(oracle64):
+    """Solid NASA7 capacity: the polynomial of the temperature's range, without the factor R
+    that the fluid applies (LBM3D/_thermal.py:191-211)."""
+    nx, steps, T0 = 160, 200, 650.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True, isPoro=True)
+    shape = (lb.nx, lb.ny, lb.nz)
+    lb.add_solid(np.full(shape, 0.5))
+    rhos = 2.5
+    lb.add_rho_solid(np.full(shape, rhos))
+    lb.set_viscosity(1e-4)
+    lb.set_fluid_thermal_diff(1e-4)
+    lb.set_fluid_Trange([300.0, 1100.0])
+    lb.set_fluid_capacity(1000.0)
+    lb.set_solid_thermal_diff_derived()
+    lb.set_solid_conductivity(0.9)
+    low = [600.0, 0.4, 0.0, 0.0, 0.0, 0.0, 0.0]
+    high = [100.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    lb.set_solid_capacity_NASA7([300.0, 1000.0, 3000.0], [low, high])
+    Trange = (300.0, 1100.0)
+    vs = Trange[1] - Trange[0]
+    base = (T0 - Trange[0]) / vs
+    x, prof, sig0 = _pulse(nx, base, 0.01 / vs)
+    lb.set_solid_Trange(list(Trange))
+    lb.boundary_condition_model = BC_MODEL.NONE
+    lb.init_field(lb.TF.S, np.full(shape, base))
+    lb.init_field(lb.TS.S, np.broadcast_to(prof[:, None, None], shape).copy())
+    lb.init_simulation()
+    lb.run(steps)
+    var = _variance(lb.TS.S.to_numpy()[:, 0, 0], base, nx)
+    D = 0.9 / rhos / _poly5_expanded(low, T0) * DTDX2
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+def test_mixture_conductivity_with_polynomial_species(oracle64):
+    """Specie.conductivity POLYNOMIAL (LBM3D/_chemical.py:120-127) is a true quartic in the fluid
+    temperature, (((c4 T + c3) T + c2) T + c1) T + c0, unlike the thermal lattices' form."""
+    nx, steps, T0 = 160, 200, 700.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True, isChemical=True)
+    lb.set_specie("A", False, 28.0)
+    lb.set_specie("B", False, 44.0)
+    ca = [0.01, 2e-5, 3e-8, 1e-11, 2e-15]
+    lb.set_specie_conductivity_poly("A", ca)
+    lb.set_specie_conductivity("B", 0.05)
+    for s in ("A", "B"):
+        lb.set_specie_capacity(s, 30.0)
+        lb.set_specie_diff(s, 1e-4)
+    YA, YB = 0.4, 0.6
+    lb.init_specie("A", YA)
+    lb.init_specie("B", YB)
+    lb.set_fluid_thermal_diff_derived()
+    lb.set_fluid_conductivity_mixture()
+    lb.set_fluid_capacity(1000.0)
+    var, sig0 = _tf_pulse_variance(lb, T0, (300.0, 1100.0), steps, nx)
+    kA = ca[0] + ca[1] * T0 + ca[2] * T0 ** 2 + ca[3] * T0 ** 3 + ca[4] * T0 ** 4
+    D = (kA * YA + 0.05 * YB) / RHO / 1000.0 * DTDX2
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
+
+
+def test_mixture_viscosity_with_sutherland_species(oracle64):
+    """Specie.viscosity SUTHERLAND (LBM3D/_chemical.py:70-77) is A T^1.5 / (T + T_s) with no
+    dt/dx^2 factor (unlike the flow's SUTHERLAND model), added as Y_j nu_j to 0.1."""
+    ny = 32
+    lb = _solver(oracle64, 2, (4, ny), isThermal=True, isChemical=True)
+    lb.set_viscosity_mixture()
+    lb.set_specie("A", False, 28.0)
+    lb.set_specie("B", False, 44.0)
+    T, Ts = 600.0, 110.4
+    A = 0.08 / (T ** 1.5 / (T + Ts))          # nu_A = 0.08 in lattice units as written
+    lb.set_specie_viscosity_sutherland("A", [A, Ts])
+    lb.set_specie_viscosity("B", 1e-4)         # 0.01 in lattice units
+    for s in ("A", "B"):
+        lb.set_specie_diff(s, 1e-4)
+    YA, YB = 0.5, 0.5
+    lb.init_specie("A", YA)
+    lb.init_specie("B", YB)
+    _thermal_uniform(lb, T)
+    nu = _shear_nu(lb, ny)
+    assert nu == pytest.approx(0.1 + YA * 0.08 + YB * 1e-4 * DTDX2, rel=0.01)
