@@ -356,14 +356,24 @@ __device__ __forceinline__ void append(bool want, unsigned idx, unsigned* list, 
     if (want) list[base + lane_prefix(bits)] = idx;
 }
 
-__device__ __forceinline__ void count3(bool act, bool hull, bool res, unsigned long long* stats) {
-    const unsigned long long a = __ballot(act), b = __ballot(hull), c = __ballot(res);
-    if ((threadIdx.x & 63) == 0 && stats) {
-        if (a) atomicAdd(stats + 0, (unsigned long long)__popcll(a));
-        if (b) atomicAdd(stats + 1, (unsigned long long)__popcll(b));
-        if (c) atomicAdd(stats + 2, (unsigned long long)__popcll(c));
+// Per-wavefront counters of evaluated / hull-fallback / residual-fallback cells, accumulated in
+// registers over a kernel's loop and added to the global statistics once at the end: a
+// same-address atomic per loop iteration of every wavefront serialises the whole device.
+struct WaveCount {
+    unsigned long long a = 0, b = 0, c = 0;
+    __device__ void add(bool act, bool hull, bool res) {
+        a += (unsigned long long)__popcll(__ballot(act));
+        b += (unsigned long long)__popcll(__ballot(hull));
+        c += (unsigned long long)__popcll(__ballot(res));
     }
-}
+    __device__ void flush(unsigned long long* stats) const {
+        if ((threadIdx.x & 63) == 0 && stats) {
+            if (a) atomicAdd(stats + 0, a);
+            if (b) atomicAdd(stats + 1, b);
+            if (c) atomicAdd(stats + 2, c);
+        }
+    }
+};
 
 // ---------------------------------------------------------------------------
 // source policies: where inputs come from and where results go
@@ -486,6 +496,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void k_plain(const lbm_chem_mech* __re
     double x[NI];
     long long cell = -1;
     bool drained = false;
+    WaveCount wc;
     for (;;) {
         const bool want = cell < 0 && !drained;
         const unsigned long long bits = __ballot(want);
@@ -509,7 +520,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void k_plain(const lbm_chem_mech* __re
                 }
             }
         }
-        count3(got, false, false, stats);
+        wc.add(got, false, false);
         if (__ballot(cell >= 0 || !drained) == 0) break;
         if (cell >= 0) {
             const int st = cell_attempt(m, dt, s);
@@ -521,6 +532,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void k_plain(const lbm_chem_mech* __re
             }
         }
     }
+    wc.flush(stats);
 }
 
 __global__ void k_zero_u32(unsigned* p, int n) {
@@ -735,6 +747,7 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     float* buf = rows[wv];
     const long long ntile = (n + SUR_BLOCK - 1) / SUR_BLOCK;
+    WaveCount wc;
     for (long long t = blockIdx.x; t < ntile; t += gridDim.x) {
         const long long cell = t * SUR_BLOCK + wv * 64 + l;
         double x[NI];
@@ -775,10 +788,11 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
         }
         if constexpr (CHECK) {
             if (list) append(hull || res, (unsigned)cell, list, count);
-            count3(act, hull, res, stats);
+            wc.add(act, hull, res);
         }
         __syncthreads();
     }
+    if constexpr (CHECK) wc.flush(stats);
 }
 
 // ---------------------------------------------------------------------------
