@@ -453,3 +453,48 @@ def test_mixture_viscosity_with_sutherland_species(oracle64):
     _thermal_uniform(lb, T)
     nu = _shear_nu(lb, ny)
     assert nu == pytest.approx(0.1 + YA * 0.08 + YB * 1e-4 * DTDX2, rel=0.01)
+
+
+def test_derived_solid_diffusivity_mixture(oracle64):
+    """Solid MIXTURE models (LBM3D/_thermal.py:177-211): kappa_s = sum over FIX species of
+    kappa_j S_j / rho_s; c_s = sum over the fluid species of (capa_j / M_j) Y_j rho (the fluid's
+    species and density, D18); rho_s = sum of the FIX species (chemistry with porous solid)."""
+    nx, steps, T0 = 160, 200, 650.0
+    lb = _solver(oracle64, 2, (nx, 4), isThermal=True, isChemical=True, isPoro=True)
+    shape = (lb.nx, lb.ny, lb.nz)
+    lb.add_solid(np.full(shape, 0.5))
+    lb.set_viscosity(1e-4)
+    for name, fix, M in (("A", False, 28.0), ("B", False, 44.0), ("C1", True, 12.0), ("C2", True, 30.0)):
+        lb.set_specie(name, fix, M)
+        lb.set_specie_diff(name, 1e-4)
+    lb.set_specie_capacity("A", 29.0)
+    lb.set_specie_capacity("B", 37.0)
+    lb.set_specie_conductivity("C1", 0.8)
+    lb.set_specie_conductivity("C2", 0.4)
+    YA, YB, S1, S2 = 0.3, 0.7, 1.2, 0.6
+    for name, v in (("A", YA), ("B", YB), ("C1", S1), ("C2", S2)):
+        lb.init_specie(name, v)
+    lb.set_fluid_thermal_diff(1e-4)
+    lb.set_fluid_Trange([300.0, 1100.0])
+    lb.set_fluid_capacity(1000.0)
+    lb.set_solid_thermal_diff_derived()
+    lb.set_solid_conductivity_mixture()
+    lb.set_solid_capacity_mixture()
+    Trange = (300.0, 1100.0)
+    vs = Trange[1] - Trange[0]
+    base = (T0 - Trange[0]) / vs
+    x, prof, sig0 = _pulse(nx, base, 0.01 / vs)
+    lb.set_solid_Trange(list(Trange))
+    lb.boundary_condition_model = BC_MODEL.NONE
+    lb.init_field(lb.rho, RHO)
+    lb.init_field(lb.TF.S, np.full(shape, base))
+    lb.init_field(lb.TS.S, np.broadcast_to(prof[:, None, None], shape).copy())
+    lb.init_simulation()
+    lb.run(steps)
+    var = _variance(lb.TS.S.to_numpy()[:, 0, 0], base, nx)
+    rhos = S1 + S2
+    kappa = (0.8 * S1 + 0.4 * S2) / rhos
+    cs = (29.0 / 0.028 * YA + 37.0 / 0.044 * YB) * RHO
+    D = kappa / rhos / cs * DTDX2
+    assert 0.005 < D < 0.5
+    assert var == pytest.approx(_expected_variance(sig0, D, steps, 2), rel=2e-5)
