@@ -720,6 +720,16 @@ __device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __r
     r[3] = acc[3];
 }
 
+// The lanes of a wavefront exchange data through the wavefront's own LDS rows (no other
+// wavefront touches them), so their accesses are ordered at wavefront scope, without a
+// workgroup barrier: wavefronts of a block then drift apart and overlap their memory and
+// matrix-core phases.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Surrogate over n cells, SUR_BLOCK cells per block iteration (64 per wavefront: every lane
 // gathers and checks one cell; the wavefront's MLP runs as four 16-cell tiles).
 // CHECK = 0: normalised in/out (lbm_mlp_forward).
@@ -761,7 +771,7 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
             for (int v = 0; v < NI; ++v) buf[l * XS + v] = act ? src.in[(long long)v * src.n + cell] : 0.0f;
         }
-        __syncthreads();
+        wave_sync();
         // the weight fragments are re-read from LDS for every tile: the empty asm with a memory
         // clobber keeps the compiler from hoisting them into (160) registers across the loop
 #pragma unroll 1
@@ -769,7 +779,7 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
             asm volatile("" ::: "memory");
             mlp_tile(wl, bl, buf + s * 16 * XS);
         }
-        __syncthreads();
+        wave_sync();
         bool hull = false, res = false;
         if (act) {
             if constexpr (CHECK) {
@@ -790,7 +800,7 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
             if (list) append(hull || res, (unsigned)cell, list, count);
             wc.add(act, hull, res);
         }
-        __syncthreads();
+        wave_sync();
     }
     if constexpr (CHECK) wc.flush(stats);
 }
