@@ -261,8 +261,12 @@ template <> struct Codec<_Float16, float> {
     static constexpr bool offset = true;
     LBM_DEV static float load(const _Float16* p, float off) { return (float)*p * (1.0f / 4096.0f) + off; }
     LBM_DEV static void store(_Float16* p, float v, float off) { *p = (_Float16)((v - off) * 4096.0f); }
-    LBM_DEV static float load_nt(const _Float16* p, float off) { return load(p, off); }
-    LBM_DEV static void store_nt(_Float16* p, float v, float off) { store(p, v, off); }
+    LBM_DEV static float load_nt(const _Float16* p, float off) {
+        return (float)__builtin_nontemporal_load(p) * (1.0f / 4096.0f) + off;
+    }
+    LBM_DEV static void store_nt(_Float16* p, float v, float off) {
+        __builtin_nontemporal_store((_Float16)((v - off) * 4096.0f), p);
+    }
 };
 // lattice weight of direction k of a QN-velocity set (19/9 flow, 7/5 scalar)
 template <int QN> LBM_DEV constexpr double lat_w(int k) {
