@@ -1268,7 +1268,7 @@ LBM_DEV void side_get_all(const DevParams<Real>& P, const Bufs<Real, Store>& B, 
 // macro, evaluated on the previous step's fields.  Only the 12 + 2 species layout carries it.
 template <class Real, class Store, int NF, int NX, class M = ModelAny>
 LBM_DEV void xsrc_add(const DevParams<Real>& P, const Bufs<Real, Store>& B, long long c, CellMac<Real, NF, NX>& m) {
-    if constexpr (NF >= 12 && NX >= 2 && !M::exact_species) {
+    if constexpr (NF >= 12 && NX >= 2) {
         if (B.xsrc) {
             const long long N = P.N;
 #pragma unroll

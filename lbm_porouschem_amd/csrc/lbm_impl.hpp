@@ -86,7 +86,8 @@ constexpr int BULK_BLOCK = LBM_BULK_BLOCK;
 static_assert(BULK_BLOCK % 64 == 0 && BULK_BLOCK <= 1024, "bulk workgroup: whole wavefronts");
 
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
-__global__ __launch_bounds__(BULK_BLOCK) LBM_WAVES_PER_EU(M::exact_species ? LBM_BULK_WAVES : 1)
+// (the 12-species coupled kernel keeps 98 scalar populations live: 2 wavefronts per SIMD)
+__global__ __launch_bounds__(BULK_BLOCK) LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? 2 : LBM_BULK_WAVES) : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
     long long c = (long long)xcd_block() * BULK_BLOCK + threadIdx.x;
@@ -621,7 +622,8 @@ struct ImplT final : Impl {
             bool sp_const = true;
             for (int j = 0; j < d.n_species; ++j)
                 if (!d.species_fix[j] && p.species[j].diff_model != LBM_DIFF_CONSTANT) sp_const = false;
-            coupled = D == 3 && NF == 4 && NX == 2 && nf == 4 && nfix == 2 && Q_.thermal && Q_.chem && Q_.poro &&
+            coupled = D == 3 && (NF == 4 || NF == 12) && NX == 2 && nf == NF && nfix == 2 && Q_.thermal && Q_.chem &&
+                      Q_.poro &&
                       !Q_.radiation && Q_.force_guo && Q_.eos == EOS_IDEAL && Q_.visc_model == VISC_CONSTANT &&
                       Q_.poro_model == PORO_DARCY && Q_.tf.diff_model == TDIFF_PRANDTL &&
                       Q_.tf.cap_model == THERMO_CONSTANT && Q_.ts.diff_model == TDIFF_CONSTANT &&
@@ -1019,7 +1021,7 @@ struct ImplT final : Impl {
 
     void launch_bulk(hipStream_t stream, bool odd, int wm) {
         const dim3 grid((unsigned)((N + BULK_BLOCK - 1) / BULK_BLOCK)), block(BULK_BLOCK);
-        if constexpr (D == 3 && NF == 4 && NX == 2) {
+        if constexpr (D == 3 && (NF == 4 || NF == 12) && NX == 2) {
             if (coupled) {
                 if (P.coll == COLL_MRT) {
                     if (odd)
