@@ -56,6 +56,12 @@ __global__ __launch_bounds__(BLOCK) void k_init(const DevParams<Real>* __restric
 #ifndef LBM_BULK_WAVES
 #define LBM_BULK_WAVES 4
 #endif
+// Odd parity alone.  At 4 wavefronts the odd-parity MRT kernels spill (FP32 36 B/lane, FP16
+// 20 B/lane); at 3 they fit (134 VGPRs).  Measured at 256^3 (mean bulk launch): FP32 1.877 ms
+// at 4 against 1.882 ms at 3, FP16 1.520 ms against 1.571 ms: occupancy matters, the spill not.
+#ifndef LBM_BULK_WAVES_ODD
+#define LBM_BULK_WAVES_ODD LBM_BULK_WAVES
+#endif
 #ifndef LBM_WAVES_PER_EU
 #define LBM_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
@@ -87,7 +93,8 @@ static_assert(BULK_BLOCK % 64 == 0 && BULK_BLOCK <= 1024, "bulk workgroup: whole
 
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
 // (the 12-species coupled kernel keeps 98 scalar populations live: 2 wavefronts per SIMD)
-__global__ __launch_bounds__(BULK_BLOCK) LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? 2 : LBM_BULK_WAVES) : 1)
+__global__ __launch_bounds__(BULK_BLOCK)
+LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? 2 : (PAR ? LBM_BULK_WAVES_ODD : LBM_BULK_WAVES)) : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
     long long c = (long long)xcd_block() * BULK_BLOCK + threadIdx.x;
