@@ -410,7 +410,7 @@ struct SrcF32 {
         for (int v = 0; v < NI; ++v) x[v] = (double)in[(long long)v * n + i];
         return true;
     }
-    __device__ void predicted(long long i, const double* x, const double* y, int flag) const {
+    __device__ void predicted(long long i, const double* x, const double* y, double, int flag) const {
 #pragma unroll
         for (int o = 0; o < NO; ++o) out[(long long)o * n + i] = (float)y[o];
         if (flags) flags[i] = flag;
@@ -448,12 +448,15 @@ template <class Real> struct SrcFields {
         x[15] = (double)f.S[(long long)(f.nf + 1) * N + i] * f.rho_ref;
         return true;
     }
-    __device__ void apply(long long i, const double* x, const double* y) const {
-        const long long N = f.N;
+    // gas density of a cell's inputs, rho_g = p / (R T sum_k Y_k / M_k)
+    __device__ double gas_density(const double* x) const {
         double sym = 0.0;
 #pragma unroll
         for (int q = 0; q < NS; ++q) sym += x[2 + q] * inv_molemass[q];
-        const double rho_g = sym > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
+        return sym > 0.0 ? x[1] / (R_GAS * x[0] * sym) : 0.0;
+    }
+    __device__ void apply(long long i, const double* x, const double* y, double rho_g) const {
+        const long long N = f.N;
         double tot = 0.0, dsum = 0.0;
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
@@ -464,16 +467,17 @@ template <class Real> struct SrcFields {
 #pragma unroll
         for (int q = 0; q < NS; ++q)
             f.xsrc[(long long)q * N + i] = (Real)(tot > 0.0 ? (rho_g * x[2 + q] + y[q]) * inv_tot - x[2 + q] : 0.0);
-        f.xsrc[12 * N + i] = (Real)(y[12] / f.rho_ref);
-        f.xsrc[13 * N + i] = (Real)((-y[12] - dsum) / f.rho_ref);
+        f.xsrc[12 * N + i] = (Real)(y[12] * f.inv_rho_ref);
+        f.xsrc[13 * N + i] = (Real)((-y[12] - dsum) * f.inv_rho_ref);
         const bool sol = f.solid[i] > (Real)0;
-        f.xsrc[14 * N + i] = sol ? (Real)0 : (Real)(y[13] / f.tf_scale);
-        f.xsrc[15 * N + i] = sol ? (Real)(y[13] / f.ts_scale) : (Real)0;
+        f.xsrc[14 * N + i] = sol ? (Real)0 : (Real)(y[13] * f.inv_tf_scale);
+        f.xsrc[15 * N + i] = sol ? (Real)(y[13] * f.inv_ts_scale) : (Real)0;
     }
-    __device__ void predicted(long long i, const double* x, const double* y, int flag) const {
-        if (!flag) apply(i, x, y);
+    // rho_g: the surrogate pass's gas density of x (the same formula and tables as gas_density)
+    __device__ void predicted(long long i, const double* x, const double* y, double rho_g, int flag) const {
+        if (!flag) apply(i, x, y, rho_g);
     }
-    __device__ void plain(long long i, const double* x, const double* y, int) const { apply(i, x, y); }
+    __device__ void plain(long long i, const double* x, const double* y, int) const { apply(i, x, y, gas_density(x)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -590,6 +594,9 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     const bf16x4h r = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
     return __builtin_bit_cast(bf16x4, r);
 }
+// max(x, 0) as one integer max on the bits: a negative float is a negative integer and becomes
+// +0 (fmaxf spends a second instruction canonicalising NaNs)
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 __device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
     const s16x8 v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     return __builtin_bit_cast(bf16x8, v);
@@ -625,29 +632,27 @@ __device__ __forceinline__ bf16x8 frag8(const unsigned short* wl, int base, int 
     return __builtin_bit_cast(bf16x8, *(const s16x8*)(wl + base + ((t * 64 + (int)(threadIdx.x & 63)) << 3)));
 }
 
-// sinh in FP32 for the output transform, branch-free: the odd Taylor series to z^7 below |z| = 1/2
-// (truncation below 1.1e-8 relative), (e - 1/e) / 2 with e = exp(|z|) from the hardware exp2 and
-// reciprocal above (the difference loses at most a factor coth(1/2) = 2.2).  Relative error about
-// 1e-6 against FP64 sinh over the outputs' range, two orders below the network's own.
-__device__ __forceinline__ float sinh_out(float z) {
-    const float a = fabsf(z), s = z * z;
-    const float series = z * (1.0f + s * (1.0f / 6.0f + s * (1.0f / 120.0f + s * (1.0f / 5040.0f))));
-    const float e = __builtin_amdgcn_exp2f(a * 1.44269504f);
-    const float big = copysignf(0.5f * (e - __builtin_amdgcn_rcpf(e)), z);
-    return a < 0.5f ? series : big;
-}
-
 // Physical outputs of one cell from the network's normalised outputs yn (lbm_surrogate: output
 // transform by out_kind, per-unit scaling by out_mul, no gas species below zero).
 // The transforms run in FP32 (the network's output is FP32); the species floor in FP64.
+// One hardware exponential per output serves every kind: e = exp(|z|) (v_exp_f32 of |z| log2 e)
+// and 1/e (v_rcp_f32) give exp(z) for kinds 1 and 3, and sinh z = (e - 1/e) / 2 for kind 2 above
+// |z| = 1/2 (the difference loses at most a factor coth(1/2) = 2.2); below it sinh is the odd
+// Taylor series to z^7 (truncation under 1.1e-8 relative).  Relative error about 1e-6 against
+// FP64 over the outputs' range, two orders below the network's own.
 __device__ void decode(const SurDev& S, const double* x, const float* yn, double rho_g, double* y) {
 #pragma unroll
     for (int o = 0; o < NO; ++o) {
         const float z = yn[o] * S.out_scale[o] + S.out_mean[o];
         const double have = o < NS ? rho_g * x[2 + o] : 0.0;
         const int kd = S.out_kind[o];
-        double v = kd == 1 ? -(double)expf(z)
-                           : (kd == 2 ? (double)(S.out_lin[o] * sinh_out(z)) : (kd == 3 ? (double)expf(z) - have : (double)z));
+        const float a = fabsf(z), q = z * z;
+        const float e = __builtin_amdgcn_exp2f(a * 1.44269504f), ei = __builtin_amdgcn_rcpf(e);
+        const float ez = z < 0.0f ? ei : e;
+        const float sh = a < 0.5f ? z * (1.0f + q * (1.0f / 6.0f + q * (1.0f / 120.0f + q * (1.0f / 5040.0f))))
+                                  : copysignf(0.5f * (e - ei), z);
+        double v = kd == 1 ? -(double)ez
+                           : (kd == 2 ? (double)(S.out_lin[o] * sh) : (kd == 3 ? (double)ez - have : (double)z));
         if (S.out_mul[o] >= 0) v *= x[S.out_mul[o]];
         if (o < NS && v < -have) v = -have;
         y[o] = v;
@@ -672,13 +677,13 @@ __device__ double residual_fast(const SurDev& S, const double* in, const double*
     double inv = 0.0, worst = 0.0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        double gas = 0.0, have = 0.0;
+        double gas = 0.0, y = 0.0;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            gas += S.eM[e][k] * out[k];
-            have += S.eM[e][k] * rho_g * in[2 + k];
+            gas = fma(S.eM[e][k], out[k], gas);
+            y = fma(S.eM[e][k], in[2 + k], y);
         }
-        have += B * S.bio_e[e];
+        const double have = fma(rho_g, y, B * S.bio_e[e]);
         double rel = -dB * S.bio_e[e];
         if (e == 0) rel -= dC * S.inv_MC;
         inv += have;
@@ -706,7 +711,7 @@ __device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __r
         const float* b = bl + mt * 16 + 4 * q;
         f32x4 acc = {b[0], b[1], b[2], b[3]};
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag4(wl, mt), xb, acc, 0, 0, 0);
-        h1[mt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+        h1[mt] = pack4(relu(acc[0]), relu(acc[1]), relu(acc[2]), relu(acc[3]));
     }
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
@@ -716,7 +721,7 @@ __device__ void mlp_tile(const unsigned short* __restrict__ wl, const float* __r
         for (int kt = 0; kt < 4; ++kt)
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag8(wl, W2_BASE, mt * 4 + kt), cat8(h1[2 * kt], h1[2 * kt + 1]),
                                                           acc, 0, 0, 0);
-        h2[mt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+        h2[mt] = pack4(relu(acc[0]), relu(acc[1]), relu(acc[2]), relu(acc[3]));
     }
     const float* b = bl + 2 * HID + 4 * q;
     f32x4 acc = {b[0], b[1], b[2], b[3]};
@@ -800,7 +805,7 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
                 for (int v = 0; v < NI; ++v)
                     if (x[v] < (double)S->lo[v] || x[v] > (double)S->hi[v]) hull = true;
                 if (!hull && residual_fast(*S, x, y, rho_g) > S->residual_tol) res = true;
-                src.predicted(cell, x, y, hull ? 1 : (res ? 2 : 0));
+                src.predicted(cell, x, y, rho_g, hull ? 1 : (res ? 2 : 0));
             } else {
 #pragma unroll
                 for (int o = 0; o < NO; ++o) src.out[(long long)o * src.n + cell] = buf[l * XS + o];

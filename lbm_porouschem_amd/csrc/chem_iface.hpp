@@ -19,6 +19,7 @@ template <class Real> struct ChemFields {
     int nf;              // fluid species on the handle (= LBM_CHEM_NSP): biomass at S[nf], char at S[nf + 1]
     double tf_scale, tf_ref, ts_scale, ts_ref;
     double p0, rho_ref, dt;
+    double inv_rho_ref, inv_tf_scale, inv_ts_scale;  // reciprocals (the per-cell passes multiply)
     Real* xsrc;          // [16][N]: 12 species dS, biomass dS, char dS, dTF, dTS (lattice units)
     unsigned* list;      // [N] fallback cells of the surrogate pass
     unsigned* count;     // [2]: fallback-list length, work-queue head

@@ -955,6 +955,9 @@ struct ImplT final : Impl {
         f.p0 = chem_p0;
         f.rho_ref = chem_rho;
         f.dt = h_->p.dt;
+        f.inv_rho_ref = 1.0 / f.rho_ref;
+        f.inv_tf_scale = 1.0 / f.tf_scale;
+        f.inv_ts_scale = 1.0 / f.ts_scale;
         f.xsrc = xsrc_buf;
         f.list = chem_list;
         f.count = chem_count;
