@@ -62,6 +62,12 @@ __global__ __launch_bounds__(BLOCK) void k_init(const DevParams<Real>* __restric
 #ifndef LBM_BULK_WAVES_ODD
 #define LBM_BULK_WAVES_ODD LBM_BULK_WAVES
 #endif
+// Lattices with more than four fluid species (98 scalar populations live at 12 species: 187-194
+// VGPRs).  At 3 wavefronts the 12-species kernels spill 72-112 B/lane; measured on the 256^3
+// 12-species lattice (mean bulk launch): 3.52 ms at 2 against 4.08 ms at 3.
+#ifndef LBM_BULK_WAVES_WIDE
+#define LBM_BULK_WAVES_WIDE 2
+#endif
 #ifndef LBM_WAVES_PER_EU
 #define LBM_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
@@ -92,9 +98,8 @@ constexpr int BULK_BLOCK = LBM_BULK_BLOCK;
 static_assert(BULK_BLOCK % 64 == 0 && BULK_BLOCK <= 1024, "bulk workgroup: whole wavefronts");
 
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
-// (the 12-species coupled kernel keeps 98 scalar populations live: 2 wavefronts per SIMD)
 __global__ __launch_bounds__(BULK_BLOCK)
-LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? 2 : (PAR ? LBM_BULK_WAVES_ODD : LBM_BULK_WAVES)) : 1)
+LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? LBM_BULK_WAVES_WIDE : (PAR ? LBM_BULK_WAVES_ODD : LBM_BULK_WAVES)) : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
     long long c = (long long)xcd_block() * BULK_BLOCK + threadIdx.x;
