@@ -89,20 +89,28 @@ __device__ __forceinline__ unsigned xcd_block() {
     return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
 }
 
-// workgroup size of the bulk kernel (a multiple of 64: each wavefront covers 64 consecutive
-// cells of the fast axis)
+// Workgroup size of the bulk kernel by population storage (a multiple of 64: each wavefront
+// covers 64 consecutive cells of the fast axis).  Measured at 256^3 (mean bulk launch, one box,
+// 64 / 128 / 256 / 512 threads): coupled FP32 MRT - / 2.078 / 1.937 / 1.941 ms; coupled FP16
+// storage 1.619 / 1.427 / 1.516 / 1.547 ms; the 12-species FP32 lattice 4.126 / 3.712 / 3.540 /
+// 3.503 ms.
 #ifndef LBM_BULK_BLOCK
 #define LBM_BULK_BLOCK 256
 #endif
-constexpr int BULK_BLOCK = LBM_BULK_BLOCK;
-static_assert(BULK_BLOCK % 64 == 0 && BULK_BLOCK <= 1024, "bulk workgroup: whole wavefronts");
+#ifndef LBM_BULK_BLOCK_F16
+#define LBM_BULK_BLOCK_F16 128
+#endif
+template <class Store> constexpr int bulk_block() { return sizeof(Store) == 2 ? LBM_BULK_BLOCK_F16 : LBM_BULK_BLOCK; }
+static_assert(LBM_BULK_BLOCK % 64 == 0 && LBM_BULK_BLOCK <= 1024 && LBM_BULK_BLOCK_F16 % 64 == 0 &&
+                  LBM_BULK_BLOCK_F16 <= 1024,
+              "bulk workgroup: whole wavefronts");
 
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
-__global__ __launch_bounds__(BULK_BLOCK)
+__global__ __launch_bounds__(bulk_block<Store>())
 LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? LBM_BULK_WAVES_WIDE : (PAR ? LBM_BULK_WAVES_ODD : LBM_BULK_WAVES)) : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
-    long long c = (long long)xcd_block() * BULK_BLOCK + threadIdx.x;
+    long long c = (long long)xcd_block() * bulk_block<Store>() + threadIdx.x;
     if (c >= P.N) return;
     bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }
@@ -1035,7 +1043,8 @@ struct ImplT final : Impl {
     }
 
     void launch_bulk(hipStream_t stream, bool odd, int wm) {
-        const dim3 grid((unsigned)((N + BULK_BLOCK - 1) / BULK_BLOCK)), block(BULK_BLOCK);
+        constexpr int bb = bulk_block<Store>();
+        const dim3 grid((unsigned)((N + bb - 1) / bb)), block(bb);
         if constexpr (D == 3 && (NF == 4 || NF == 12) && NX == 2) {
             if (coupled) {
                 if (P.coll == COLL_MRT) {
