@@ -802,9 +802,9 @@ __global__ __launch_bounds__(SUR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
                 const double rho_g = gas_density(*S, x);
                 decode(*S, x, buf + l * XS, rho_g, y);
 #pragma unroll
-                for (int v = 0; v < NI; ++v)
-                    if (x[v] < (double)S->lo[v] || x[v] > (double)S->hi[v]) hull = true;
-                if (!hull && residual_fast(*S, x, y, rho_g) > S->residual_tol) res = true;
+                for (int v = 0; v < NI; ++v)  // written so that a NaN input or residual falls back
+                    if (!(x[v] >= (double)S->lo[v] && x[v] <= (double)S->hi[v])) hull = true;
+                if (!hull && !(residual_fast(*S, x, y, rho_g) <= S->residual_tol)) res = true;
                 src.predicted(cell, x, y, rho_g, hull ? 1 : (res ? 2 : 0));
             } else {
 #pragma unroll

@@ -497,7 +497,7 @@ int chem_oracle_surrogate_cell(const lbm_chem* c, const double* in, double dt, i
     int flag = 0;
     float xn[NI];
     for (int i = 0; i < NI; ++i) {
-        if (in[i] < (double)s->hull_lo[i] || in[i] > (double)s->hull_hi[i]) flag = 1;
+        if (!(in[i] >= (double)s->hull_lo[i] && in[i] <= (double)s->hull_hi[i])) flag = 1; /* NaN: outside */
         xn[i] = ((float)in[i] - s->in_mean[i]) * s->in_scale[i];
     }
     mlp_trace t;
@@ -520,7 +520,7 @@ int chem_oracle_surrogate_cell(const lbm_chem* c, const double* in, double dt, i
         if (o < NS && v < -have) v = -have;
         out[o] = v;
     }
-    if (!flag && lbm_chem_residual(&c->mech, in, out) > s->residual_tol) flag = 2;
+    if (!flag && !(lbm_chem_residual(&c->mech, in, out) <= s->residual_tol)) flag = 2;
     if (flag && fallback) chem_oracle_cell(&c->mech, in, dt, out);
     return flag;
 }

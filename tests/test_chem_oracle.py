@@ -430,3 +430,24 @@ def test_surrogate_output_transforms_invert_training_targets(oracle64):
         # inputs pass through float32 on the way in, so the per-unit output scales with float32(x)
         ref[12] = U[i] * float(np.float32(X[14, i]))
         np.testing.assert_allclose(y[:, 0], ref, rtol=5e-6, atol=0)
+
+
+def test_nan_inputs_fall_out_of_the_hull(oracle64):
+    """The hull test is an in-range test, so a NaN input (which compares false both ways) is
+    outside the hull instead of inside it, and a finite cell next to it is unaffected."""
+    rng = np.random.default_rng(5)
+    n = 12
+    x = chem.cell_inputs(rng.uniform(950, 1050, n), P_ATM * np.ones(n), np.repeat(syngas_air()[:, None], n, 1),
+                         np.ones(n), np.zeros(n)).astype(np.float32)
+    x[0, 3] = np.nan
+    x[14, 8] = np.nan
+    C = chem.Chem(oracle64)
+    C.set_surrogate({"params": np.zeros(chem.NPARAM, np.float32), "in_mean": np.zeros(16, np.float32),
+                     "in_scale": np.ones(16, np.float32), "out_mean": np.zeros(14, np.float32),
+                     "out_scale": np.ones(14, np.float32), "hull_lo": np.full(16, -1e30, np.float32),
+                     "hull_hi": np.full(16, 1e30, np.float32), "residual_tol": 1e30})
+    out = np.zeros((14, n), np.float32)
+    flags = np.zeros(n, np.int32)
+    C.surrogate(x, out, flags, dt=1e-5, fallback=0)
+    assert flags[3] == 1 and flags[8] == 1
+    assert (flags == 0).sum() == n - 2

@@ -171,6 +171,31 @@ def test_surrogate_checks_and_fallback(gchem, ochem):
     np.testing.assert_allclose(got[:, fb], p[:, fb], rtol=1e-6, atol=1e-30)
 
 
+def test_surrogate_nan_input_falls_back(gchem, ochem):
+    """A NaN input leaves the hull on the device as in the checker (the checks are in-range
+    tests); fallback off, so no cell is integrated."""
+    n = 256
+    x = random_states(n, 17)
+    sur = toy_surrogate(x)
+    sur["hull_lo"] = (x.min(axis=1) - 1e9).astype(np.float32)
+    sur["hull_hi"] = (x.max(axis=1) + 1e9).astype(np.float32)
+    sur["residual_tol"] = 1e30
+    gchem.set_surrogate(sur)
+    ochem.set_surrogate(sur)
+    xf = x.astype(np.float32)
+    xf[0, 7] = np.nan
+    xf[5, 70] = np.nan
+    xf[15, 200] = np.nan
+    ref = np.zeros((14, n), dtype=np.float32)
+    rflags = np.zeros(n, dtype=np.int32)
+    ochem.surrogate(xf, ref, rflags, dt=1e-5, fallback=0)
+    flags = dev(np.zeros(n, dtype=np.int32))
+    gchem.surrogate(dev(xf), dev(np.zeros((14, n), dtype=np.float32)), flags, dt=1e-5, fallback=0)
+    gf = host(flags)
+    assert list(np.flatnonzero(rflags)) == [7, 70, 200]
+    np.testing.assert_array_equal(gf, rflags)
+
+
 def test_surrogate_output_transforms_match_checker(gchem, ochem):
     """The device decode of the output transforms (asinh-scaled gas and heat outputs, log-space
     biomass conversion per unit of biomass) against the checker's, cell by cell."""
