@@ -223,8 +223,8 @@ def main():
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU slab decomposition is a "
-              "design only in this build.", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU slab decomposition is "
+              "designed but not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import torch  # device selection / presence check only

@@ -150,7 +150,7 @@ def main():
     ap.add_argument("--time-steps-plain", type=int, default=3)
     ap.add_argument("--time-steps-surrogate", type=int, default=20)
     ap.add_argument("--load", default="", help="skip data collection and training: load this surrogate")
-    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--out", default="surrogate_study_out", help="directory for surrogate.npz and study.json")
     args = ap.parse_args()
     from .. import _abi
     lib = _abi.load_library()
