@@ -13,7 +13,7 @@
  * reference's step unchanged.
  *
  * Exported by liblbm_mi355x.so (HIP kernels) and, for the entry points marked
- * [oracle], by the CPU checker liblbm_oracle.so with identical semantics.
+ * [oracle], by the CPU oracle liblbm_oracle.so with identical semantics.
  *
  * Cell state (the 16 operator inputs, in this order):
  *   0      T      temperature [K]

@@ -4,7 +4,7 @@
  *
  * Two shared libraries export exactly this interface:
  *   liblbm_mi355x.so   the product: HIP kernels for gfx950 (lbm_porouschem_amd/csrc)
- *   liblbm_oracle.so   the CPU restatement used only as the test checker (oracle/)
+ *   liblbm_oracle.so   the CPU restatement used only as the test oracle (oracle/)
  *
  * The reference exposes no FFI; its boundary is the Python solver-class API that
  * src/LBM/app.py drives.  Each entry point below replaces one piece of that API

@@ -4,7 +4,7 @@
 gfx950) that sits next to this file.  There is no fallback: if the library is
 missing the import of the solver fails with an error naming the build step.
 `CLib` can also wrap another library exporting the same ABI (the tests use this
-to drive the CPU checker under oracle/ with identical calls).
+to drive the CPU oracle under oracle/ with identical calls).
 """
 import ctypes
 import os

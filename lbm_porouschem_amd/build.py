@@ -24,15 +24,15 @@ def hipcc():
 
 
 # translation units: (source, extra flags).  The FP64 kernels are compiled without
-# floating-point contraction so they round like the CPU checker.  The FP32 kernels
+# floating-point contraction so they round like the CPU oracle.  The FP32 kernels
 # use the hardware reciprocal for division and square root (about 1 ulp instead of
-# correctly rounded): their error budget against the FP64 checker is 1e-5 relative.
+# correctly rounded): their error budget against the FP64 oracle is 1e-5 relative.
 UNITS = [
     ("lbm_abi.hip", []),
     ("lbm_f64.hip", ["-ffp-contract=off"]),
     ("lbm_f32.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("lbm_f16.hip", ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
-    # chemistry operator: FP64 integrator rounded like the CPU checker; MLP kernels
+    # chemistry operator: FP64 integrator rounded like the CPU oracle; MLP kernels
     ("chem.hip", ["-ffp-contract=off"]),
     ("ensemble.hip", []),
 ]

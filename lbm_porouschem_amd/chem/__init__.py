@@ -2,7 +2,7 @@
 mechanism / pyrolysis tables it runs on.
 
 `Chem(lib)` wraps one chemistry context of a library exporting the ABI: the HIP
-library (arrays are device tensors, torch as plumbing) or the CPU checker under
+library (arrays are device tensors, torch as plumbing) or the CPU oracle under
 oracle/ (arrays are numpy).  `syngas_mechanism()` builds the default operator
 description: the H2/CO gas-phase sub-mechanism of GRI-Mech 3.0 (12 species, 40
 reactions, generated by make_mechanism.py) plus a three-route biomass pyrolysis.
@@ -203,7 +203,7 @@ def bind(lib):
 
 class Chem:
     """One chemistry context of `clib` (an _abi.CLib).  Array arguments are numpy arrays for
-    the CPU checker and torch tensors on the device for the HIP library, structure of
+    the CPU oracle and torch tensors on the device for the HIP library, structure of
     arrays: value v of cell i at [v, i]."""
 
     def __init__(self, clib, mech=None, device=0):

@@ -4,7 +4,7 @@
 // lattice handle runs after every step when the operator is attached.
 //
 // Built with -ffp-contract=off: the FP64 integrator then rounds operation by
-// operation like the CPU checker.
+// operation like the CPU oracle.
 //
 // Kernels
 //   k_plain<Src>      persistent lanes over a work queue of cells: gather inputs, two-stage
@@ -53,7 +53,7 @@ constexpr int SUR_WAVES = 8;
 constexpr int SUR_BLOCK = 64 * SUR_WAVES;  // surrogate: 64 cells per wavefront per iteration
 
 // ---------------------------------------------------------------------------
-// FP64 operator (same algorithm and operation order as the CPU checker)
+// FP64 operator (same algorithm and operation order as the CPU oracle)
 // ---------------------------------------------------------------------------
 struct TCache {
     double kf[NR], k0[NR], lfc[NR], ikc[NR];
@@ -193,7 +193,7 @@ __device__ void cell_start(const lbm_chem_mech& m, const double* in, double dt, 
 }
 
 // One ROS2 attempt.  Returns 0 while the cell is unfinished, 1 when it reached dt, -1 when it
-// hit max_steps (same step control and Jacobian reuse as the CPU checker).
+// hit max_steps (same step control and Jacobian reuse as the CPU oracle).
 __device__ int cell_attempt(const lbm_chem_mech& m, double dt, Cell& s) {
     const double gam = 1.0 + 1.0 / sqrt(2.0);
     const int last = s.h >= dt - s.t;

@@ -1,5 +1,5 @@
 // lbm_f64.hip -- FP64 instantiations of the step kernels.  Built with
-// -ffp-contract=off so the FP64 path rounds like the CPU checker (parity <= 1e-10
+// -ffp-contract=off so the FP64 path rounds like the CPU oracle (parity <= 1e-10
 // also where the reference's eps + 1e-12 denominators make feq ill-conditioned).
 #include "lbm_impl.hpp"
 

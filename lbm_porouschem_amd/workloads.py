@@ -5,7 +5,7 @@ temperature, gray porous disks); `coupled3d` is the 3D coupled case (256^3,
 FP32, D3Q19 flow, D3Q7 fluid/solid temperature and 4 fluid species, 2 FIX
 species, three competing Arrhenius pyrolysis reactions, gray porous spheres).
 Both return a configured solver ready for init_simulation(); the same builder
-drives the HIP library or, in the tests, the CPU checker (``lib=``).
+drives the HIP library or, in the tests, the CPU oracle (``lib=``).
 
 The physical units follow the reference's convention (lattice constants
 derived from dx, dt: viscosity * dt / dx^2, LBM3D/_io.py:121-171): dx = dt = 1e-4

@@ -1,6 +1,6 @@
 /*
  * chem_oracle.c -- CPU restatement of the cell-local chemistry operator declared in
- * include/lbm_chem.h.  TEST INFRASTRUCTURE: this file is built into the CPU checker
+ * include/lbm_chem.h.  TEST INFRASTRUCTURE: this file is built into the CPU oracle
  * (liblbm_oracle*.so); only the tests and the bench's CPU baseline load it.
  *
  * The reference has no such operator (its reaction source is the explicit Arrhenius
@@ -542,7 +542,7 @@ int lbm_chem_surrogate(lbm_chem* c, const float* in, float* out, int* flags, lon
     return LBM_OK;
 }
 
-/* Checker-only export: the operator's right-hand side at state y = (c[12] mol/m^3, T K,
+/* Oracle-only export: the operator's right-hand side at state y = (c[12] mol/m^3, T K,
  * rho_B kg/m^3) with solid heat capacity Csol (J/(m^3 K)); f receives dy/dt.  The tests
  * integrate it with library ODE solvers and check conservation laws on it. */
 int lbm_chem_oracle_rhs(const lbm_chem_mech* m, const double* y, double Csol, double* f) {

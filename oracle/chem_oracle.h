@@ -1,4 +1,4 @@
-/* chem_oracle.h -- internals shared by the CPU checker's lattice code (lbm_oracle.c)
+/* chem_oracle.h -- internals shared by the CPU oracle's lattice code (lbm_oracle.c)
  * and its chemistry operator (chem_oracle.c).  TEST INFRASTRUCTURE: only the tests
  * and the bench's CPU baseline load the library these files build. */
 #ifndef CHEM_ORACLE_H

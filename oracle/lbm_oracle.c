@@ -1,7 +1,7 @@
 /*
  * lbm_oracle.c -- CPU restatement of the reference's coupled LBM step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests compare
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the parity tests compare
  * the HIP library against, and the "port" CPU baseline that bench.py times.
  * It is never linked into, loaded by, or used as a fallback of the product
  * library (lbm_porouschem_amd/).
@@ -1476,7 +1476,7 @@ int lbm_profile_read(lbm_handle* h, int kernel, double* total_ms, long long* lau
     *launches = 0;
     return LBM_OK;
 }
-/* Checker-only accessor (not part of the product ABI): populations of one lattice
+/* Oracle-only accessor (not part of the product ABI): populations of one lattice
  * after the last step, in the reference's layout.  lattice 0 = f (flow), 1 = TF.g,
  * 2 = TS.g, 3 + j = g of species j; values of direction k for all cells. */
 int lbm_oracle_pops(lbm_handle* h, int lattice, int k, double* out, size_t n) {
@@ -1496,7 +1496,7 @@ int lbm_oracle_pops(lbm_handle* h, int lattice, int k, double* out, size_t n) {
     for (size_t i = 0; i < n; ++i) out[i] = (double)src[i * Q + k];
     return LBM_OK;
 }
-/* Checker-only: velocity set of the flow lattice (e[k][0..2]) and weights. */
+/* Oracle-only: velocity set of the flow lattice (e[k][0..2]) and weights. */
 int lbm_oracle_lattice(lbm_handle* h, int scalar, int* e, double* w, int* lr) {
     if (!h) return LBM_ERR_INVALID;
     int Q = scalar ? h->Qs : h->Q;

@@ -1,5 +1,5 @@
 """Small seeded configurations shared by the parity tests (built identically for the
-HIP library and the CPU checker)."""
+HIP library and the CPU oracle)."""
 import numpy as np
 
 from lbm_porouschem_amd.flag import BC, BC_FLOW, FORCE_TERM

@@ -1,6 +1,6 @@
 """CPU tests of the callers around the step: geometry voxelisation and surface
 weights, the VTK writer, the YAML application loop, and snapshot / resume.  The
-application is driven through the CPU checker library so no GPU is needed (the
+application is driven through the CPU oracle library so no GPU is needed (the
 product library runs the same code path on the GPU in test_gpu_app.py)."""
 import json
 import os

@@ -1,4 +1,4 @@
-"""Pinning of the CPU checker's chemistry operator (oracle/chem_oracle.c) without the
+"""Pinning of the CPU oracle's chemistry operator (oracle/chem_oracle.c) without the
 product library: library ODE solvers, chemical equilibrium by Gibbs minimisation,
 conservation laws, closed forms of the pyrolysis and fall-off limits, and torch
 autograd for the MLP gradient.

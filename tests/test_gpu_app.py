@@ -9,7 +9,7 @@ from test_app_io import _config
 pytestmark = pytest.mark.gpu
 
 
-def test_application_matches_checker(tmp_path, hip, oracle64):
+def test_application_matches_oracle(tmp_path, hip, oracle64):
     print("[gpu app] tiny 2D configuration fp64", flush=True)
     cfg = _config(tmp_path)
     A = app.application(cfg, lib=hip, precision="fp64")

@@ -1,5 +1,5 @@
 """Parity of the HIP chemistry operator (lbm_porouschem_amd/csrc/chem.hip) with the CPU
-checker (oracle/chem_oracle.c): FP64 integrator, BF16 matrix-core MLP forward, training
+oracle (oracle/chem_oracle.c): FP64 integrator, BF16 matrix-core MLP forward, training
 gradient and Adam update, surrogate checks with fallback, and the coupled step with the
 operator attached.  All calls go through the C ABI of include/lbm_chem.h."""
 import numpy as np
@@ -51,7 +51,7 @@ def ochem(oracle64):
     return chem.Chem(oracle64)
 
 
-def test_integrate_matches_checker(gchem, ochem):
+def test_integrate_matches_oracle(gchem, ochem):
     n = 200
     x = random_states(n, 11)
     ref = np.zeros((14, n))
@@ -88,7 +88,7 @@ def random_params(seed, scale=0.15):
 
 
 @pytest.mark.parametrize("n", [1, 63, 1000])
-def test_mfma_forward_matches_checker(gchem, ochem, n):
+def test_mfma_forward_matches_oracle(gchem, ochem, n):
     """The BF16 v_mfma tile layout: any transposed operand or misplaced lane shows up here."""
     P = random_params(21)
     xn = np.random.default_rng(n).standard_normal((16, n)).astype(np.float32)
@@ -101,7 +101,7 @@ def test_mfma_forward_matches_checker(gchem, ochem, n):
     assert np.max(np.abs(got - ref)) <= 2e-2 * np.max(np.abs(ref))
 
 
-def test_training_gradient_matches_checker(gchem, ochem):
+def test_training_gradient_matches_oracle(gchem, ochem):
     n = 777
     rng = np.random.default_rng(7)
     P = random_params(22)
@@ -115,7 +115,7 @@ def test_training_gradient_matches_checker(gchem, ochem):
     assert np.linalg.norm(host(g) - g_ref) <= 1e-4 * np.linalg.norm(g_ref)
 
 
-def test_adam_matches_checker(gchem, ochem):
+def test_adam_matches_oracle(gchem, ochem):
     rng = np.random.default_rng(8)
     P = rng.standard_normal(chem.NPARAM).astype(np.float32)
     Pd, md, vd = dev(P.copy()), dev(np.zeros_like(P)), dev(np.zeros_like(P))
@@ -172,7 +172,7 @@ def test_surrogate_checks_and_fallback(gchem, ochem):
 
 
 def test_surrogate_nan_input_falls_back(gchem, ochem):
-    """A NaN input leaves the hull on the device as in the checker (the checks are in-range
+    """A NaN input leaves the hull on the device as in the oracle (the checks are in-range
     tests); fallback off, so no cell is integrated."""
     n = 256
     x = random_states(n, 17)
@@ -196,9 +196,9 @@ def test_surrogate_nan_input_falls_back(gchem, ochem):
     np.testing.assert_array_equal(gf, rflags)
 
 
-def test_surrogate_output_transforms_match_checker(gchem, ochem):
+def test_surrogate_output_transforms_match_oracle(gchem, ochem):
     """The device decode of the output transforms (asinh-scaled gas and heat outputs, log-space
-    biomass conversion per unit of biomass) against the checker's, cell by cell."""
+    biomass conversion per unit of biomass) against the oracle's, cell by cell."""
     n = 500
     x = random_states(n, 17)
     sur = toy_surrogate(x, seed=33)

@@ -1,7 +1,7 @@
 """Parity of the HIP library against the CPU restatement of the reference step.
 
 FP64 kernels: relative L2 <= 1e-10 on every macroscopic field after each block of
-steps.  FP32 kernels: relative L2 <= 1e-5 against the FP64 checker on velocity,
+steps.  FP32 kernels: relative L2 <= 1e-5 against the FP64 oracle on velocity,
 temperatures, species and solid density after 20 steps (and <= 2e-5 after 100)."""
 import numpy as np
 import pytest
@@ -42,7 +42,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_fp64_matches_checker(name, hip, oracle64):
+def test_fp64_matches_oracle(name, hip, oracle64):
     kw = CASES[name]
     print(f"[parity fp64] {name} {kw}", flush=True)
     A = build(hip, prec="fp64", **kw)
@@ -58,7 +58,7 @@ def test_fp64_matches_checker(name, hip, oracle64):
 
 
 @pytest.mark.parametrize("name", ["2d_thermal_gray", "3d_thermal_gray", "3d_chem", "3d_fastface_chem"])
-def test_fp32_against_fp64_checker(name, hip, oracle64):
+def test_fp32_against_fp64_oracle(name, hip, oracle64):
     kw = CASES[name]
     print(f"[parity fp32] {name} {kw}", flush=True)
     A = build(hip, prec="fp32", **kw)
@@ -77,7 +77,7 @@ def test_fp32_against_fp64_checker(name, hip, oracle64):
         assert rel_l2(fa[k], fb[k]) <= 2e-5, (name, 100, k, rel_l2(fa[k], fb[k]))
 
 
-def test_reductions_match_checker(hip, oracle64):
+def test_reductions_match_oracle(hip, oracle64):
     from lbm_porouschem_amd import _abi
     kw = CASES["3d_chem"]
     A = build(hip, prec="fp64", **kw)
@@ -98,7 +98,7 @@ def test_reductions_match_checker(hip, oracle64):
 
 @pytest.mark.parametrize("collision", ["mrt", "bgk"])
 @pytest.mark.parametrize("nz", [16, 64])
-def test_coupled_workload_matches_checker(nz, collision, hip, oracle64):
+def test_coupled_workload_matches_oracle(nz, collision, hip, oracle64):
     """The 3D coupled pyrolysis setup runs the specialised kernel (compile-time model
     switches, flow lattice finished before the scalar lattices); nz = 64 also runs the
     z-face boundary inside the bulk kernel."""
@@ -117,10 +117,10 @@ def test_coupled_workload_matches_checker(nz, collision, hip, oracle64):
 
 
 # FP16 population storage (deviation from the lattice reference, scaled by 2^12), FP32
-# arithmetic: every field within 1e-3 relative L2 of the FP64 checker after 100 steps
+# arithmetic: every field within 1e-3 relative L2 of the FP64 oracle after 100 steps
 # (the measured figures are ~1e-4, a few units of FP16's 4.9e-4 half-ulp).
 @pytest.mark.parametrize("name", ["2d_thermal_gray", "3d_chem", "3d_mrt_chem", "3d_fastface_chem"])
-def test_fp16_storage_against_fp64_checker(name, hip, oracle64):
+def test_fp16_storage_against_fp64_oracle(name, hip, oracle64):
     kw = CASES[name]
     print(f"[parity fp16] {name} {kw}", flush=True)
     A = build(hip, prec="fp16", **kw)
@@ -150,10 +150,10 @@ def test_fp16_coupled_trace_species(hip, oracle64):
         assert rel_l2(fa[k], fb[k]) <= 1e-3, (k, rel_l2(fa[k], fb[k]))
 
 
-def test_plain2d_full_run_matches_checker(hip, oracle64):
+def test_plain2d_full_run_matches_oracle(hip, oracle64):
     """The 2D plain case at its full size and length: 256x128, FP64, D2Q9 BGK (tau = 0.8)
     + D2Q5 fluid and solid temperature, gray disks, inlet/outflow/walls, 5000 steps,
-    compared with the checker every 1000 steps (relative L2 <= 1e-10 on every field)."""
+    compared with the oracle every 1000 steps (relative L2 <= 1e-10 on every field)."""
     from lbm_porouschem_amd import workloads
     print("[parity plain2d] 256x128 fp64, 5000 steps", flush=True)
     A = workloads.plain2d(lib=hip)

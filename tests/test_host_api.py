@@ -183,7 +183,7 @@ def test_init_field_rounds_through_float32():
 
 
 def test_product_path_has_no_cpu_fallback(monkeypatch, tmp_path):
-    # the solver refuses to run without the HIP library instead of routing to the CPU checker
+    # the solver refuses to run without the HIP library instead of routing to the CPU oracle
     monkeypatch.setattr(_abi, "_PRODUCT", None)
     monkeypatch.setattr(_abi, "library_path", lambda: str(tmp_path / "liblbm_mi355x.so"))
     with pytest.raises(ImportError):

@@ -1,4 +1,4 @@
-"""Pins the MRT collision of the CPU checker (oracle/lbm_oracle.c, mrt_collide).
+"""Pins the MRT collision of the CPU oracle (oracle/lbm_oracle.c, mrt_collide).
 
 The reference is BGK only (LBM3D/_evolution.py:33), so the multiple-relaxation-time
 collision has no reference counterpart; it is fixed by what the method itself
