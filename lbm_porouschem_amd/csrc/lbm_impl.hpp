@@ -192,12 +192,12 @@ __global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict
     nee_item<Real, Store, D, NF, NX>(P, B, family, nparts, rows, nrows, t);
 }
 
-template <class Real, class Store, int D, int NF, int NX, int PAR>
+template <class Real, class Store, int D, int NF, int NX, int PAR, class M = ModelAny>
 __global__ __launch_bounds__(BLOCK) void k_face(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B) {
     const DevParams<Real>& P = *Pp;
     long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= B.nflist) return;
-    face_collide_cell<Real, Store, D, NF, NX, PAR>(P, B, t);
+    face_collide_cell<Real, Store, D, NF, NX, PAR, M>(P, B, t);
 }
 
 
@@ -917,14 +917,7 @@ struct ImplT final : Impl {
         }
         ev_end(1, e_nee);
         hipEvent_t e_face = eager ? ev_begin(2) : nullptr;
-        if (nflist) {
-            if (stepno & 1)
-                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1>), dim3(grid_for(nflist)), dim3(BLOCK), 0, st, dP,
-                                   B);
-            else
-                hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0>), dim3(grid_for(nflist)), dim3(BLOCK), 0, st, dP,
-                                   B);
-        }
+        if (nflist) launch_face(st, (stepno & 1) != 0);
         ev_end(2, e_face);
         if (eager && chem_mode) {
             int rc = run_chem();
@@ -1069,6 +1062,31 @@ struct ImplT final : Impl {
             hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelAny>), grid, block, 0, stream, dP, B, wm);
         else
             hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelAny>), grid, block, 0, stream, dP, B, wm);
+    }
+
+    // the face kernel runs the same model policy as the bulk kernel
+    void launch_face(hipStream_t stream, bool odd) {
+        const dim3 grid((unsigned)grid_for(nflist)), block(BLOCK);
+        if constexpr (D == 3 && (NF == 4 || NF == 12) && NX == 2) {
+            if (coupled) {
+                if (P.coll == COLL_MRT) {
+                    if (odd)
+                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1, ModelCoupledMRT>), grid, block, 0, stream, dP, B);
+                    else
+                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0, ModelCoupledMRT>), grid, block, 0, stream, dP, B);
+                } else {
+                    if (odd)
+                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1, ModelCoupled>), grid, block, 0, stream, dP, B);
+                    else
+                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0, ModelCoupled>), grid, block, 0, stream, dP, B);
+                }
+                return;
+            }
+        }
+        if (odd)
+            hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1, ModelAny>), grid, block, 0, stream, dP, B);
+        else
+            hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0, ModelAny>), grid, block, 0, stream, dP, B);
     }
 
     int reduce(int kind, int comp, double* out) override {
