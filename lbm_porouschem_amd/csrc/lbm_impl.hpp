@@ -101,6 +101,20 @@ __device__ __forceinline__ unsigned xcd_block() {
 #define LBM_BULK_BLOCK_F16 128
 #endif
 template <class Store> constexpr int bulk_block() { return sizeof(Store) == 2 ? LBM_BULK_BLOCK_F16 : LBM_BULK_BLOCK; }
+// With LBM_XCD_PAIR_F16=1 (the default) the FP16 kernels renumber each group of 16 blocks so that blocks b and
+// b + 8, which the dispatcher places on the same XCD, take adjacent tiles: with 128-thread
+// workgroups the two halves of a 256-cell row then share one L2 (the odd parity's line straddled
+// at the split is fetched once).  A bijection on every full group of 16; the tail keeps its order.
+// Measured at 256^3 coupled FP16 (mean bulk launch, three passes): 1.405 ms against 1.420 ms.
+#ifndef LBM_XCD_PAIR_F16
+#define LBM_XCD_PAIR_F16 1
+#endif
+template <class Store> __device__ __forceinline__ unsigned bulk_tile() {
+    const unsigned b = xcd_block();
+    if (sizeof(Store) != 2 || !LBM_XCD_PAIR_F16 || b >= gridDim.x / 16u * 16u) return b;
+    const unsigned r = b % 16u;
+    return b - r + (r % 8u) * 2u + r / 8u;
+}
 static_assert(LBM_BULK_BLOCK % 64 == 0 && LBM_BULK_BLOCK <= 1024 && LBM_BULK_BLOCK_F16 % 64 == 0 &&
                   LBM_BULK_BLOCK_F16 <= 1024,
               "bulk workgroup: whole wavefronts");
@@ -110,7 +124,7 @@ __global__ __launch_bounds__(bulk_block<Store>())
 LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? LBM_BULK_WAVES_WIDE : (PAR ? LBM_BULK_WAVES_ODD : LBM_BULK_WAVES)) : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
-    long long c = (long long)xcd_block() * bulk_block<Store>() + threadIdx.x;
+    long long c = (long long)bulk_tile<Store>() * bulk_block<Store>() + threadIdx.x;
     if (c >= P.N) return;
     bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }
