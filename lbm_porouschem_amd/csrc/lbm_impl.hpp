@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -1049,58 +1050,41 @@ struct ImplT final : Impl {
         return LBM_OK;
     }
 
-    void launch_bulk(hipStream_t stream, bool odd, int wm) {
-        constexpr int bb = bulk_block<Store>();
-        const dim3 grid((unsigned)((N + bb - 1) / bb)), block(bb);
+    // Calls f(parity, policy) with the parity as std::integral_constant<int, 0 or 1> and the model
+    // policy the bulk and face kernels run: the coupled policy (BGK or MRT) where `coupled` holds,
+    // the generic one elsewhere.
+    template <class F> void with_policy(bool odd, F&& f) {
+        using P0 = std::integral_constant<int, 0>;
+        using P1 = std::integral_constant<int, 1>;
         if constexpr (D == 3 && (NF == 4 || NF == 12) && NX == 2) {
             if (coupled) {
                 if (P.coll == COLL_MRT) {
-                    if (odd)
-                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelCoupledMRT>), grid, block, 0, stream, dP,
-                                           B, wm);
-                    else
-                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelCoupledMRT>), grid, block, 0, stream, dP,
-                                           B, wm);
+                    if (odd) f(P1{}, ModelCoupledMRT{}); else f(P0{}, ModelCoupledMRT{});
                 } else {
-                    if (odd)
-                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelCoupled>), grid, block, 0, stream, dP, B,
-                                           wm);
-                    else
-                        hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelCoupled>), grid, block, 0, stream, dP, B,
-                                           wm);
+                    if (odd) f(P1{}, ModelCoupled{}); else f(P0{}, ModelCoupled{});
                 }
                 return;
             }
         }
-        if (odd)
-            hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 1, ModelAny>), grid, block, 0, stream, dP, B, wm);
-        else
-            hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, 0, ModelAny>), grid, block, 0, stream, dP, B, wm);
+        if (odd) f(P1{}, ModelAny{}); else f(P0{}, ModelAny{});
+    }
+
+    void launch_bulk(hipStream_t stream, bool odd, int wm) {
+        constexpr int bb = bulk_block<Store>();
+        const dim3 grid((unsigned)((N + bb - 1) / bb)), block(bb);
+        with_policy(odd, [&](auto par, auto model) {
+            hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, decltype(par)::value, decltype(model)>), grid, block, 0,
+                               stream, dP, B, wm);
+        });
     }
 
     // the face kernel runs the same model policy as the bulk kernel
     void launch_face(hipStream_t stream, bool odd) {
         const dim3 grid((unsigned)grid_for(nflist)), block(BLOCK);
-        if constexpr (D == 3 && (NF == 4 || NF == 12) && NX == 2) {
-            if (coupled) {
-                if (P.coll == COLL_MRT) {
-                    if (odd)
-                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1, ModelCoupledMRT>), grid, block, 0, stream, dP, B);
-                    else
-                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0, ModelCoupledMRT>), grid, block, 0, stream, dP, B);
-                } else {
-                    if (odd)
-                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1, ModelCoupled>), grid, block, 0, stream, dP, B);
-                    else
-                        hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0, ModelCoupled>), grid, block, 0, stream, dP, B);
-                }
-                return;
-            }
-        }
-        if (odd)
-            hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 1, ModelAny>), grid, block, 0, stream, dP, B);
-        else
-            hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, 0, ModelAny>), grid, block, 0, stream, dP, B);
+        with_policy(odd, [&](auto par, auto model) {
+            hipLaunchKernelGGL((k_face<Real, Store, D, NF, NX, decltype(par)::value, decltype(model)>), grid, block, 0,
+                               stream, dP, B);
+        });
     }
 
     int reduce(int kind, int comp, double* out) override {
