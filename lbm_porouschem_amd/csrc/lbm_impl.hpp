@@ -146,7 +146,7 @@ __host__ __device__ inline long long nee_items(int family, int nx, int ny, int n
 }
 
 template <class Real, class Store, int D, int NF, int NX>
-__device__ void nee_item(const DevParams<Real>& P, const Bufs<Real, Store>& B, int family, int nparts,
+__device__ void nee_item(const DevParams<Real>& P, const Bufs<Real, Store>& B, int family,
                          const uint32_t* __restrict__ rows, long long nrows, long long t) {
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const bool fastfam = family == (D == 3 ? 2 : 1);
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(BLOCK) void k_nee(const DevParams<Real>* __restrict
     const DevParams<Real>& P = *Pp;
     const long long t = (long long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= nee_items<D>(family, P.nx, P.ny, P.nz, nparts, nrows)) return;
-    nee_item<Real, Store, D, NF, NX>(P, B, family, nparts, rows, nrows, t);
+    nee_item<Real, Store, D, NF, NX>(P, B, family, rows, nrows, t);
 }
 
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M = ModelAny>
