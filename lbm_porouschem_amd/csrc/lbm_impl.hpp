@@ -101,7 +101,13 @@ __device__ __forceinline__ unsigned xcd_block() {
 #ifndef LBM_BULK_BLOCK_F16
 #define LBM_BULK_BLOCK_F16 128
 #endif
-template <class Store> constexpr int bulk_block() { return sizeof(Store) == 2 ? LBM_BULK_BLOCK_F16 : LBM_BULK_BLOCK; }
+// lattices with more than four fluid species (2 wavefronts per SIMD) use LBM_BULK_BLOCK_WIDE
+#ifndef LBM_BULK_BLOCK_WIDE
+#define LBM_BULK_BLOCK_WIDE 512
+#endif
+template <class Store, int NF> constexpr int bulk_block() {
+    return sizeof(Store) == 2 ? LBM_BULK_BLOCK_F16 : (NF > 4 ? LBM_BULK_BLOCK_WIDE : LBM_BULK_BLOCK);
+}
 // With LBM_XCD_PAIR_F16=1 (the default) the FP16 kernels renumber each group of 16 blocks so that blocks b and
 // b + 8, which the dispatcher places on the same XCD, take adjacent tiles: with 128-thread
 // workgroups the two halves of a 256-cell row then share one L2 (the odd parity's line straddled
@@ -117,15 +123,15 @@ template <class Store> __device__ __forceinline__ unsigned bulk_tile() {
     return b - r + (r % 8u) * 2u + r / 8u;
 }
 static_assert(LBM_BULK_BLOCK % 64 == 0 && LBM_BULK_BLOCK <= 1024 && LBM_BULK_BLOCK_F16 % 64 == 0 &&
-                  LBM_BULK_BLOCK_F16 <= 1024,
+                  LBM_BULK_BLOCK_F16 <= 1024 && LBM_BULK_BLOCK_WIDE % 64 == 0 && LBM_BULK_BLOCK_WIDE <= 1024,
               "bulk workgroup: whole wavefronts");
 
 template <class Real, class Store, int D, int NF, int NX, int PAR, class M>
-__global__ __launch_bounds__(bulk_block<Store>())
+__global__ __launch_bounds__((bulk_block<Store, NF>()))
 LBM_WAVES_PER_EU(M::exact_species ? (NF > 4 ? LBM_BULK_WAVES_WIDE : (PAR ? LBM_BULK_WAVES_ODD : LBM_BULK_WAVES)) : 1)
 void k_bulk(const DevParams<Real>* __restrict__ Pp, Bufs<Real, Store> B, int write_macro) {
     const DevParams<Real>& P = *Pp;
-    long long c = (long long)bulk_tile<Store>() * bulk_block<Store>() + threadIdx.x;
+    long long c = (long long)bulk_tile<Store>() * bulk_block<Store, NF>() + threadIdx.x;
     if (c >= P.N) return;
     bulk_cell<Real, Store, D, NF, NX, PAR, M>(P, B, c, write_macro);
 }
@@ -1070,7 +1076,7 @@ struct ImplT final : Impl {
     }
 
     void launch_bulk(hipStream_t stream, bool odd, int wm) {
-        constexpr int bb = bulk_block<Store>();
+        constexpr int bb = bulk_block<Store, NF>();
         const dim3 grid((unsigned)((N + bb - 1) / bb)), block(bb);
         with_policy(odd, [&](auto par, auto model) {
             hipLaunchKernelGGL((k_bulk<Real, Store, D, NF, NX, decltype(par)::value, decltype(model)>), grid, block, 0,
